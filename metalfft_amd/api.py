@@ -1,0 +1,195 @@
+"""Python mirror of MetalFFT's public FFT API over the libmifft C-ABI.
+
+Names, argument meaning and error behaviour follow the Swift interface
+(docs/MetalFFT.swiftinterface:217-247 in the reference):
+
+    MetalFFT(size)                      MetalFFT.init(size:)           MetalFFT.swift:27-51
+    .forward(x)                         forward(_:)                    MetalFFT.swift:69-85
+    .forward_into(input, output)        forward(input:output:)         MetalFFT.swift:88-104
+    .forward_batch(batch)               forward(batch:)                MetalFFT.swift:108-133
+    .inverse(x)                         inverse(_:)                    MetalFFT.swift:56-65
+    MultiChannelFFT(channels, size)     MultiChannelFFT                MultiChannelFFT.swift:19-54
+    FFTError                            FFTError                       MetalContext.swift:6-30
+
+Host arrays are numpy complex64 (interleaved re/im, the SIMD2<Float> layout).
+``execute`` additionally runs on device buffers (torch tensors on the GPU),
+asynchronously on a HIP stream, which the Swift API has no counterpart for.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import FORWARD, INVERSE, FFTError
+
+SUPPORTED_SIZES = (64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
+
+
+def _as_c64(x) -> np.ndarray:
+    a = np.asarray(x)
+    if a.dtype == np.complex64:
+        return np.ascontiguousarray(a.reshape(-1))
+    if np.iscomplexobj(a):
+        return np.ascontiguousarray(a.reshape(-1).astype(np.complex64))
+    if a.ndim >= 1 and a.shape[-1] == 2 and a.dtype.kind == "f":
+        # (N, 2) real pairs, like [SIMD2<Float>]
+        return np.ascontiguousarray(a.astype(np.float32)).view(np.complex64).reshape(-1)
+    return np.ascontiguousarray(a.reshape(-1).astype(np.complex64))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(_lib.fptr)
+
+
+def device_name() -> str:
+    L = _lib.load()
+    buf = ctypes.create_string_buffer(256)
+    _lib.check(L.mifft_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def kernel_variants(n: int) -> list[str]:
+    L = _lib.load()
+    cnt = L.mifft_kernel_variants(n, None, 0)
+    arr = (ctypes.c_char_p * max(cnt, 1))()
+    L.mifft_kernel_variants(n, arr, cnt)
+    return [arr[i].decode() for i in range(cnt)]
+
+
+def _stream_handle(stream) -> ctypes.c_void_p:
+    if stream is None:
+        try:
+            import torch
+            stream = torch.cuda.current_stream()
+        except Exception:  # no torch: null stream
+            return ctypes.c_void_p(0)
+    h = getattr(stream, "cuda_stream", stream)
+    return ctypes.c_void_p(int(h))
+
+
+def _dev_ptr(t) -> int:
+    if hasattr(t, "data_ptr"):
+        if not t.is_cuda:
+            raise FFTError(11, "device entry point needs a GPU tensor")
+        if not t.is_contiguous():
+            raise FFTError(11, "device tensor must be contiguous")
+        return int(t.data_ptr())
+    return int(t)
+
+
+class MetalFFT:
+    """Batched complex FFT of one length (64 ... 16384, powers of two)."""
+
+    def __init__(self, size: int):
+        L = _lib.load()
+        self._plan = ctypes.c_void_p()
+        _lib.check(L.mifft_plan_create(int(size), ctypes.byref(self._plan)))
+        self.size = int(size)
+
+    # -- plan control -------------------------------------------------------
+    @property
+    def kernel_name(self) -> str:
+        return _lib.load().mifft_plan_kernel_name(self._plan).decode()
+
+    def select_kernel(self, name: str) -> None:
+        _lib.check(_lib.load().mifft_plan_select_kernel(self._plan, name.encode()))
+
+    def close(self) -> None:
+        if getattr(self, "_plan", None) and self._plan.value:
+            _lib.load().mifft_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- host API (blocking, copies; reference semantics) -------------------
+    def forward(self, x) -> np.ndarray:
+        a = _as_c64(x)
+        out = np.empty(self.size, dtype=np.complex64)
+        _lib.check(_lib.load().mifft_forward_host(self._plan, _fp(a.view(np.float32)), a.size,
+                                                  _fp(out.view(np.float32))))
+        return out
+
+    def forward_into(self, input, output: np.ndarray | None) -> np.ndarray:
+        """forward(input:output:): writes into ``output`` when it already
+        holds ``size`` complex64 values, else returns a new array (the
+        reference resizes its inout array, MetalFFT.swift:95)."""
+        a = _as_c64(input)
+        if (output is None or not isinstance(output, np.ndarray) or output.dtype != np.complex64
+                or output.size != self.size or not output.flags.c_contiguous):
+            if a.size != self.size:  # same check order as the reference
+                self.forward(a)
+            output = np.empty(self.size, dtype=np.complex64)
+        _lib.check(_lib.load().mifft_forward_host(self._plan, _fp(a.view(np.float32)), a.size,
+                                                  _fp(output.view(np.float32))))
+        return output
+
+    def inverse(self, x) -> np.ndarray:
+        a = _as_c64(x)
+        out = np.empty(self.size, dtype=np.complex64)
+        _lib.check(_lib.load().mifft_inverse_host(self._plan, _fp(a.view(np.float32)), a.size,
+                                                  _fp(out.view(np.float32))))
+        return out
+
+    def _batch(self, batch: Sequence, fn) -> list[np.ndarray]:
+        if len(batch) == 0:
+            return []
+        arrs = [_as_c64(b) for b in batch]
+        outs = [np.empty(self.size, dtype=np.complex64) for _ in arrs]
+        n = len(arrs)
+        ins = (_lib.fptr * n)(*[_fp(a.view(np.float32)) for a in arrs])
+        ous = (_lib.fptr * n)(*[_fp(o.view(np.float32)) for o in outs])
+        cnt = (ctypes.c_int64 * n)(*[a.size for a in arrs])
+        _lib.check(fn(self._plan, ins, cnt, n, ous))
+        return outs
+
+    def forward_batch(self, batch: Sequence) -> list[np.ndarray]:
+        return self._batch(batch, _lib.load().mifft_forward_batch_host)
+
+    def inverse_batch(self, batch: Sequence) -> list[np.ndarray]:
+        return self._batch(batch, _lib.load().mifft_inverse_batch_host)
+
+    # -- device API -----------------------------------------------------------
+    def execute(self, src, dst, batch: int | None = None, direction: int = FORWARD,
+                stream=None) -> None:
+        """Transform ``batch`` contiguous length-``size`` complex64 signals
+        already on the GPU (torch tensors or raw device addresses),
+        asynchronously on ``stream`` (default: torch's current stream)."""
+        if batch is None:
+            batch = src.numel() // self.size
+            if hasattr(src, "dtype") and not src.is_complex():
+                batch = src.numel() // (2 * self.size)
+        _lib.check(_lib.load().mifft_execute_device(self._plan, _dev_ptr(src), _dev_ptr(dst),
+                                                    int(batch), int(direction),
+                                                    _stream_handle(stream)))
+
+
+class MultiChannelFFT:
+    """Same-size FFTs over several channels in one launch per call
+    (Sources/MetalFFT/FFT/MultiChannelFFT.swift:19-54); the inverse is one
+    batched launch too instead of the reference's per-channel loop (:49-53)."""
+
+    def __init__(self, channels: int, size: int):
+        if channels <= 0:
+            raise FFTError(10, "channels must be > 0")
+        self.channels = int(channels)
+        self.size = int(size)
+        self._fft = MetalFFT(size)
+
+    def _check(self, inputs):
+        if len(inputs) != self.channels:
+            raise FFTError(10, f"Expected {self.channels} channels, got {len(inputs)}")
+
+    def forward(self, inputs: Sequence) -> list[np.ndarray]:
+        self._check(inputs)
+        return self._fft.forward_batch(inputs)
+
+    def inverse(self, inputs: Sequence) -> list[np.ndarray]:
+        self._check(inputs)
+        return self._fft.inverse_batch(inputs)

@@ -1,0 +1,167 @@
+"""Pins the CPU oracle (oracle/cpu_fft.c + FP64 numpy) before it is trusted.
+
+Checks, each able to catch a plausible slip (sign, index, dropped term,
+transposed operand):
+  * numpy FP64 FFT vs the brute-force O(N^2) DFT;
+  * closed-form known answers from the reference's tests: impulse -> ones
+    (Tests/MetalFFTTests/MetalFFTTests.swift:11-23), DC -> N*delta
+    (src/fft_host.swift:527-531), cos at bin 100 (MetalFFTTests.swift:37-49),
+    the three-tone signal of src/fft_host.swift:409-421;
+  * FP32 restatement vs FP64 at every size, radix-8 vs radix-4, four-step;
+  * inverse round trip (Tests/MetalFFTTests/CrossSpectralTests.swift:49-63);
+  * overlap-add convolution vs direct FP64 convolution, delta filter
+    identity (Tests/MetalFFTTests/FusedConvolverTests.swift:10-21);
+  * correlation vs the direct lag sum r[m] = sum_n a[(n+m) mod N] b[n];
+  * committed golden vectors (tests/golden/fft_golden.npz).
+"""
+import numpy as np
+import pytest
+
+from conftest import uniform_c64
+
+SIZES = (64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512])
+def test_numpy_fp64_matches_brute_force(oracle, n):
+    x = uniform_c64((n,), 7).astype(np.complex128)
+    assert oracle.rel_l2(oracle.f64_fft(x), oracle.dft_brute_f64(x)) < 1e-13
+    assert oracle.rel_l2(oracle.dft_brute_f64(x, inverse=True) / n, oracle.f64_ifft(x)) < 1e-13
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_impulse_gives_ones(oracle, n):
+    x = np.zeros(n, np.complex64)
+    x[0] = 1
+    y = oracle.cpu_fft(x)
+    assert np.max(np.abs(y - 1)) < 1e-6
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_dc_gives_scaled_delta(oracle, n):
+    y = oracle.cpu_fft(np.ones(n, np.complex64))
+    expect = np.zeros(n)
+    expect[0] = n
+    assert np.max(np.abs(y - expect)) < 1e-6 * n
+
+
+def test_cosine_peak_bin(oracle):
+    n = 1024
+    k = np.arange(n)
+    x = np.cos(2 * np.pi * 100 * k / n).astype(np.complex64)
+    y = oracle.cpu_fft(x)
+    assert int(np.argmax(np.abs(y[: n // 2]))) == 100
+    assert abs(y[100] - n / 2) < 1e-3 and abs(y[n - 100] - n / 2) < 1e-3
+
+
+def test_three_tone_closed_form(oracle):
+    # x = cos(2pi 100k/N) + 0.5 cos(2pi 200k/N) + 0.3 sin(2pi 500k/N)
+    n = 4096
+    t = 2 * np.pi * np.arange(n) / n
+    x = (np.cos(100 * t) + 0.5 * np.cos(200 * t) + 0.3 * np.sin(500 * t)).astype(np.complex64)
+    expect = np.zeros(n, np.complex128)
+    expect[100] = expect[n - 100] = n / 2
+    expect[200] = expect[n - 200] = n / 4
+    expect[500] = -0.3j * n / 2
+    expect[n - 500] = 0.3j * n / 2
+    for radix8 in (False, True):
+        y = oracle.cpu_fft(x, radix8=radix8)
+        assert oracle.rel_l2(y, expect) < 1e-6
+
+
+def test_sign_error_is_caught(oracle):
+    """A transform with the wrong exponent sign (conjugated twiddles) fails
+    the three-tone known answer: the check above can see that mistake."""
+    n = 4096
+    t = 2 * np.pi * np.arange(n) / n
+    x = (np.cos(100 * t) + 0.3 * np.sin(500 * t)).astype(np.complex64)
+    wrong = np.conj(oracle.cpu_fft(np.conj(x)))  # = sum x e^{+2 pi i nk/N}
+    expect = np.zeros(n, np.complex128)
+    expect[100] = expect[n - 100] = n / 2
+    expect[500] = -0.3j * n / 2
+    expect[n - 500] = 0.3j * n / 2
+    assert oracle.rel_l2(wrong, expect) > 0.1
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_fp32_restatement_vs_fp64(oracle, n):
+    x = uniform_c64((3, n), n)
+    y = oracle.cpu_fft(x)
+    assert oracle.rel_l2(y, oracle.f64_fft(x)) < 1e-6
+    z = oracle.cpu_fft(y, inverse=True)
+    assert oracle.rel_l2(z, x) < 1e-6
+
+
+def test_radix8_matches_radix4(oracle):
+    x = uniform_c64((4, 4096), 3)
+    a = oracle.cpu_fft(x, radix8=False)
+    b = oracle.cpu_fft(x, radix8=True)
+    assert oracle.rel_l2(b, a) < 1e-6
+    assert oracle.rel_l2(b, oracle.f64_fft(x)) < 1e-6
+
+
+@pytest.mark.parametrize("n", [8192, 16384])
+def test_four_step_index_identity(oracle, n):
+    # impulse at position p -> X[k] = W_N^{pk}: pins the four-step's
+    # transpose order and twiddle exponent (MetalFFT.swift:206-277)
+    p = 3 * n // 7
+    x = np.zeros(n, np.complex64)
+    x[p] = 1
+    k = np.arange(n)
+    expect = np.exp(-2j * np.pi * ((p * k) % n) / n)
+    assert oracle.max_abs(oracle.cpu_fft(x), expect) < 1e-5
+
+
+def test_twiddle_table(oracle):
+    for n in (64, 4096, 16384):
+        tw = oracle.twiddle_table(n)
+        exact = np.exp(-2j * np.pi * np.arange(n) / n)
+        # rounded once from FP64: within half an FP32 ulp of the exact value
+        assert np.max(np.abs(tw.real - exact.real)) <= 2 ** -25
+        assert np.max(np.abs(tw.imag - exact.imag)) <= 2 ** -25
+
+
+def test_golden_vectors(oracle, golden):
+    for n in SIZES:
+        x, X = golden[f"x_{n}"], golden[f"X_{n}"]
+        assert oracle.rel_l2(oracle.f64_fft(x), X) < 1e-14
+        assert oracle.rel_l2(oracle.cpu_fft(x), X) < 1e-6
+    x = uniform_c64((256, 4096), 0)
+    assert np.allclose(np.sum(np.abs(oracle.f64_fft(x)) ** 2, -1), golden["cfg1_energy"], rtol=1e-12)
+
+
+def test_convolve_matches_direct(oracle):
+    rng = np.random.default_rng(5)
+    s = rng.standard_normal(9000).astype(np.float32)
+    t = rng.standard_normal(129).astype(np.float32)
+    y = oracle.cpu_convolve(s, t, 1024)
+    direct = np.convolve(s.astype(np.float64), t.astype(np.float64))
+    assert y.size == s.size + t.size - 1
+    assert oracle.rel_l2(y, direct) < 1e-6
+    assert oracle.rel_l2(oracle.f64_convolve(s, t), direct) < 1e-12
+
+
+def test_convolve_delta_identity(oracle):
+    s = np.random.default_rng(6).uniform(-1, 1, 4033).astype(np.float32)
+    d = np.zeros(64, np.float32)
+    d[0] = 1
+    y = oracle.cpu_convolve(s, d, 4096)
+    assert oracle.rel_l2(y[: s.size], s) < 1e-6
+
+
+def test_convolve_rejects_long_kernel(oracle):
+    with pytest.raises(ValueError):
+        oracle.cpu_convolve(np.ones(10, np.float32), np.ones(64, np.float32), 64)
+
+
+def test_correlate_matches_lag_sum(oracle):
+    rng = np.random.default_rng(8)
+    n = 256
+    a = rng.standard_normal(300)  # truncated to n (Correlate.swift:54)
+    b = rng.standard_normal(100)  # zero-padded
+    pa = a[:n]
+    pb = np.zeros(n)
+    pb[:100] = b
+    direct = np.array([sum(pa[(i + m) % n] * pb[i] for i in range(n)) for m in range(n)])
+    assert oracle.rel_l2(oracle.f64_correlate(a, b, n), direct) < 1e-12
+    assert oracle.rel_l2(oracle.cpu_correlate(a, b, n), direct) < 1e-6
