@@ -94,6 +94,14 @@ def load():
         raise ImportError(
             f"libmifft.so not found at {LIB_PATH}; build it with "
             "`python -m metalfft_amd.build` (hipcc, gfx950)")
+    # torch ships its own libamdhip64 / libhsa-runtime64.  Two HIP runtimes
+    # in one process cannot both open the GPU, so when torch is present it is
+    # loaded first and libmifft's libamdhip64.so.7 dependency binds to the
+    # runtime already in the process (same soname).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for name, res, args in _SIGNATURES:
         fn = getattr(L, name)
