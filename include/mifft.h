@@ -64,6 +64,25 @@ typedef enum mifft_direction {
     MIFFT_INVERSE = 1   /* conj(FFT(conj x)) / N */
 } mifft_direction;
 
+/* Arithmetic of a plan.  The MFMA forms run every radix-16 stage of the
+ * N = 4096 transform as a DFT-16 GEMM on the matrix cores with FP32
+ * accumulation (the reference's CT-MMA variant, src/fft_4096_ct_mma.metal:88-206,
+ * does this with 8x8 simdgroup matrices); SPLIT adds the hi/lo product
+ * F_lo x_hi + F_hi x_lo + F_hi x_hi.  No reference counterpart for FP16/BF16. */
+typedef enum mifft_precision {
+    MIFFT_PREC_FP32 = 0,            /* FP32 Stockham on the VALU (all sizes) */
+    MIFFT_PREC_FP16_MFMA = 1,       /* n = 4096 */
+    MIFFT_PREC_BF16_MFMA = 2,       /* n = 4096 */
+    MIFFT_PREC_FP16_MFMA_SPLIT = 3, /* n = 4096 */
+    MIFFT_PREC_BF16_MFMA_SPLIT = 4  /* n = 4096 */
+} mifft_precision;
+
+/* Element type in device memory. */
+typedef enum mifft_io {
+    MIFFT_IO_C64 = 0, /* complex64: interleaved FP32 (re, im), 8 bytes */
+    MIFFT_IO_C32H = 1 /* complex FP16: interleaved FP16 (re, im), 4 bytes (MFMA plans only) */
+} mifft_io;
+
 /* ------------------------------------------------------------- library */
 /* Text of the FFTError description for a status (MetalContext.swift:17-29). */
 MIFFT_API const char *mifft_status_string(mifft_status s);
@@ -87,6 +106,11 @@ typedef struct mifft_plan mifft_plan;
 /* MetalFFT.init(size:) (MetalFFT.swift:27-51): n in {64, ..., 16384} powers of
  * two, else MIFFT_ERR_UNSUPPORTED_SIZE (FFTDescriptor, MetalContext.swift:48-67). */
 MIFFT_API mifft_status mifft_plan_create(int64_t n, mifft_plan **plan);
+/* Plan with an explicit arithmetic and device element type; the host entry
+ * points below accept complex64 plans only.  MFMA precisions and complex-FP16
+ * I/O exist for n = 4096 (else MIFFT_ERR_KERNEL_NOT_FOUND). */
+MIFFT_API mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io io,
+                                            mifft_plan **plan);
 MIFFT_API mifft_status mifft_plan_destroy(mifft_plan *plan);
 /* MetalFFT.size */
 MIFFT_API int64_t mifft_plan_size(const mifft_plan *plan);
@@ -125,6 +149,104 @@ MIFFT_API mifft_status mifft_forward_batch_host(mifft_plan *plan, const float *c
  * inverse per channel, Sources/MetalFFT/FFT/MultiChannelFFT.swift:49-53). */
 MIFFT_API mifft_status mifft_inverse_batch_host(mifft_plan *plan, const float *const *inputs,
                                       const int64_t *counts, int64_t batch, float *const *outputs);
+
+/* ------------------------------------ convolver (Convolver / FusedConvolver) */
+typedef struct mifft_convolver mifft_convolver;
+
+/* FusedConvolver.Precision (Sources/MetalFFT/DSP/FusedConvolver.swift:27-36). */
+typedef enum mifft_conv_precision {
+    MIFFT_CONV_FP32 = 0,         /* .float32 */
+    MIFFT_CONV_FP16_PURE = 1,    /* .float16Pure    (not built: MIFFT_ERR_KERNEL_NOT_FOUND) */
+    MIFFT_CONV_FP16_STORAGE = 2, /* .float16Storage (not built: MIFFT_ERR_KERNEL_NOT_FOUND) */
+    MIFFT_CONV_FP16_MIXED = 3    /* .float16Mixed   (not built: MIFFT_ERR_KERNEL_NOT_FOUND) */
+} mifft_conv_precision;
+
+/* Convolver.init(kernel:fftSize:) (Convolve.swift:19-32) with fused = 0, or
+ * FusedConvolver.init(kernel:fftSize:precision:) (FusedConvolver.swift:55-86)
+ * with fused = 1.  fft_size must be a supported FFT size greater than k
+ * (MIFFT_ERR_PRECONDITION otherwise; the reference traps), and 4096 when
+ * fused.  The taps' spectrum H = FFT(taps zero-padded) is computed on the
+ * device once.  Both forms run the same one-launch fused kernel. */
+MIFFT_API mifft_status mifft_convolver_create(const float *taps, int64_t k, int64_t fft_size,
+                                              int fused, int precision, mifft_convolver **cv);
+MIFFT_API mifft_status mifft_convolver_destroy(mifft_convolver *cv);
+/* blockSize, hopSize (= fftSize - K + 1), kernelLen */
+MIFFT_API int64_t mifft_convolver_block_size(const mifft_convolver *cv);
+MIFFT_API int64_t mifft_convolver_hop_size(const mifft_convolver *cv);
+MIFFT_API int64_t mifft_convolver_kernel_len(const mifft_convolver *cv);
+/* Convolver.recommendedFFTSize(forKernelCount:) (Convolve.swift:83-86):
+ * smallest supported size > kernel_count, or -1 (Swift nil). */
+MIFFT_API int64_t mifft_convolver_recommended_fft_size(int64_t kernel_count);
+/* apply(to:) (Convolve.swift:36-76, FusedConvolver.swift:95-133): full linear
+ * convolution of one host signal, out has samples + K - 1 floats. Blocking. */
+MIFFT_API mifft_status mifft_convolver_apply_host(mifft_convolver *cv, const float *signal,
+                                                  int64_t samples, float *out);
+/* `count` signals of `samples` floats each (contiguous) on the device ->
+ * out[count][samples + K - 1], one launch, asynchronous on `stream`. */
+MIFFT_API mifft_status mifft_convolver_apply_device(const mifft_convolver *cv, const float *signals,
+                                                    int64_t samples, int64_t count, float *out,
+                                                    void *stream);
+/* Replaces the taps (K device floats) and recomputes H on `stream`. */
+MIFFT_API mifft_status mifft_convolver_set_taps_device(mifft_convolver *cv, const float *taps,
+                                                       void *stream);
+/* Adjoint of apply_device (no reference counterpart): from grad_out
+ * [count][samples + K - 1] writes grad_signal[count][samples] (correlation of
+ * grad_out with the taps) and grad_taps[K] (sum over signals of the
+ * correlation of grad_out with the signal).  Signal spectra are recomputed,
+ * not stored; the tap sum is reduced in a fixed order (deterministic). */
+MIFFT_API mifft_status mifft_convolver_backward_device(mifft_convolver *cv, const float *signals,
+                                                       int64_t samples, int64_t count,
+                                                       const float *grad_out, float *grad_signal,
+                                                       float *grad_taps, void *stream);
+
+/* ------------------------------------------------------------ correlator */
+typedef struct mifft_correlator mifft_correlator;
+
+/* Correlator.init(fftSize:) (Sources/MetalFFT/DSP/Correlate.swift:16-23). */
+MIFFT_API mifft_status mifft_correlator_create(int64_t fft_size, mifft_correlator **c);
+MIFFT_API mifft_status mifft_correlator_destroy(mifft_correlator *c);
+MIFFT_API int64_t mifft_correlator_fft_size(const mifft_correlator *c);
+/* cross(_:_:) (Correlate.swift:33-49): out[m] = sum_n a[(n+m) mod N] b[n]
+ * with a, b truncated or zero-padded to N (:53-61); out has N floats. */
+MIFFT_API mifft_status mifft_correlator_cross_host(mifft_correlator *c, const float *a, int64_t na,
+                                                   const float *b, int64_t nb, float *out);
+/* auto(_:) (Correlate.swift:27-29) = cross(x, x). */
+MIFFT_API mifft_status mifft_correlator_auto_host(mifft_correlator *c, const float *x, int64_t n,
+                                                  float *out);
+/* `count` correlations on the device: a[count][na], b[count][nb] ->
+ * out[count][N], one launch, asynchronous on `stream`. */
+MIFFT_API mifft_status mifft_correlator_cross_device(const mifft_correlator *c, const float *a,
+                                                     int64_t na, const float *b, int64_t nb,
+                                                     int64_t count, float *out, void *stream);
+
+/* ---------------------------------------------------------- ragged batches */
+/* Transforms of different lengths in one launch (no reference counterpart:
+ * MetalFFT.forward(batch:) rejects uneven lengths, MetalFFT.swift:108-114). */
+typedef struct mifft_ragged_plan mifft_ragged_plan;
+
+/* Execution mode bits for mifft_ragged_execute_device. */
+#define MIFFT_RAGGED_PERSISTENT 0 /* one persistent launch over all lengths */
+#define MIFFT_RAGGED_PER_SIZE 1   /* one launch per distinct length (baseline) */
+#define MIFFT_RAGGED_GRAPH 2      /* capture the launch(es) in a hipGraph once per
+                                     (in, out, direction, mode) and replay it */
+
+/* lengths[i] in {64, ..., 16384} (else MIFFT_ERR_UNSUPPORTED_SIZE with
+ * batch_index = i); offsets[i] = first complex element of transform i, or
+ * NULL for back-to-back packing in the given order. */
+MIFFT_API mifft_status mifft_ragged_plan_create(const int64_t *lengths, const int64_t *offsets,
+                                                int64_t count, mifft_ragged_plan **plan);
+MIFFT_API mifft_status mifft_ragged_plan_destroy(mifft_ragged_plan *plan);
+/* Complex elements the buffers must hold: max(offset + length). */
+MIFFT_API int64_t mifft_ragged_extent(const mifft_ragged_plan *plan);
+/* Sum of lengths (for byte / flop accounting). */
+MIFFT_API int64_t mifft_ragged_elements(const mifft_ragged_plan *plan);
+/* Kernel launches one execution issues in `mode` (1, or the distinct lengths). */
+MIFFT_API int64_t mifft_ragged_launches(const mifft_ragged_plan *plan, int mode);
+/* Forward or inverse of every transform, complex64 device buffers (in == out
+ * allowed), asynchronous on `stream`.  Not thread-safe per plan. */
+MIFFT_API mifft_status mifft_ragged_execute_device(mifft_ragged_plan *plan, const float *in,
+                                                   float *out, mifft_direction dir, int mode,
+                                                   void *stream);
 
 #ifdef __cplusplus
 }

@@ -5,8 +5,12 @@ in include/mifft.h); this package is its Python mirror of MetalFFT's Swift
 API.  Importing does not touch the GPU; the first plan creation does.
 """
 from ._lib import FFTError, FORWARD, INVERSE  # noqa: F401
-from .api import (MetalFFT, MultiChannelFFT, SUPPORTED_SIZES,  # noqa: F401
+from .api import (MetalFFT, MultiChannelFFT, PrecisionFFT, RaggedFFT,  # noqa: F401
+                  RAGGED_GRAPH, RAGGED_PER_SIZE, RAGGED_PERSISTENT, SUPPORTED_SIZES,
                   device_name, kernel_variants)
+from .dsp import Convolver, Correlator, FusedConvolver, Precision  # noqa: F401
 
 __all__ = ["FFTError", "FORWARD", "INVERSE", "MetalFFT", "MultiChannelFFT",
-           "SUPPORTED_SIZES", "device_name", "kernel_variants"]
+           "SUPPORTED_SIZES", "device_name", "kernel_variants", "Convolver", "FusedConvolver",
+           "Correlator", "Precision", "RaggedFFT", "PrecisionFFT", "RAGGED_GRAPH",
+           "RAGGED_PER_SIZE", "RAGGED_PERSISTENT"]

@@ -66,6 +66,7 @@ _SIGNATURES = [
     ("mifft_shutdown", c_int, []),
     ("mifft_device_name", c_int, [ctypes.c_char_p, c_size_t]),
     ("mifft_plan_create", c_int, [c_int64, ctypes.POINTER(c_void_p)]),
+    ("mifft_plan_create_ex", c_int, [c_int64, c_int, c_int, ctypes.POINTER(c_void_p)]),
     ("mifft_plan_destroy", c_int, [c_void_p]),
     ("mifft_plan_size", c_int64, [c_void_p]),
     ("mifft_plan_kernel_name", c_char_p, [c_void_p]),
@@ -76,6 +77,30 @@ _SIGNATURES = [
     ("mifft_inverse_host", c_int, [c_void_p, fptr, c_int64, fptr]),
     ("mifft_forward_batch_host", c_int, [c_void_p, fpp, i64p, c_int64, fpp]),
     ("mifft_inverse_batch_host", c_int, [c_void_p, fpp, i64p, c_int64, fpp]),
+    ("mifft_convolver_create", c_int, [fptr, c_int64, c_int64, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    ("mifft_convolver_destroy", c_int, [c_void_p]),
+    ("mifft_convolver_block_size", c_int64, [c_void_p]),
+    ("mifft_convolver_hop_size", c_int64, [c_void_p]),
+    ("mifft_convolver_kernel_len", c_int64, [c_void_p]),
+    ("mifft_convolver_recommended_fft_size", c_int64, [c_int64]),
+    ("mifft_convolver_apply_host", c_int, [c_void_p, fptr, c_int64, fptr]),
+    ("mifft_convolver_apply_device", c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    ("mifft_convolver_set_taps_device", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("mifft_convolver_backward_device", c_int,
+     [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("mifft_ragged_plan_create", c_int, [i64p, i64p, c_int64, ctypes.POINTER(c_void_p)]),
+    ("mifft_ragged_plan_destroy", c_int, [c_void_p]),
+    ("mifft_ragged_extent", c_int64, [c_void_p]),
+    ("mifft_ragged_elements", c_int64, [c_void_p]),
+    ("mifft_ragged_launches", c_int64, [c_void_p, c_int]),
+    ("mifft_ragged_execute_device", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    ("mifft_correlator_create", c_int, [c_int64, ctypes.POINTER(c_void_p)]),
+    ("mifft_correlator_destroy", c_int, [c_void_p]),
+    ("mifft_correlator_fft_size", c_int64, [c_void_p]),
+    ("mifft_correlator_cross_host", c_int, [c_void_p, fptr, c_int64, fptr, c_int64, fptr]),
+    ("mifft_correlator_auto_host", c_int, [c_void_p, fptr, c_int64, fptr]),
+    ("mifft_correlator_cross_device", c_int,
+     [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
 ]
 
 _lib = None

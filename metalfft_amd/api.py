@@ -193,3 +193,84 @@ class MultiChannelFFT:
     def inverse(self, inputs: Sequence) -> list[np.ndarray]:
         self._check(inputs)
         return self._fft.inverse_batch(inputs)
+
+
+RAGGED_PERSISTENT = 0
+RAGGED_PER_SIZE = 1
+RAGGED_GRAPH = 2
+
+
+class RaggedFFT:
+    """Transforms of different lengths (64 ... 16384) in one launch.
+
+    ``lengths[i]`` complex values of transform i start at ``offsets[i]``
+    (default: packed back to back).  ``execute`` runs all of them forward or
+    inverse on device buffers; ``mode`` picks one persistent launch
+    (default), one launch per length, and/or hipGraph replay."""
+
+    def __init__(self, lengths: Sequence[int], offsets: Sequence[int] | None = None):
+        n = len(lengths)
+        L = _lib.load()
+        ln = (ctypes.c_int64 * max(n, 1))(*[int(v) for v in lengths])
+        of = None
+        if offsets is not None:
+            of = (ctypes.c_int64 * max(n, 1))(*[int(v) for v in offsets])
+        self._p = ctypes.c_void_p()
+        _lib.check(L.mifft_ragged_plan_create(ln, of, n, ctypes.byref(self._p)))
+        self.count = n
+        self.extent = int(L.mifft_ragged_extent(self._p))
+        self.elements = int(L.mifft_ragged_elements(self._p))
+
+    def launches(self, mode: int = RAGGED_PERSISTENT) -> int:
+        return int(_lib.load().mifft_ragged_launches(self._p, int(mode)))
+
+    def execute(self, src, dst, direction: int = FORWARD, mode: int = RAGGED_PERSISTENT,
+                stream=None) -> None:
+        _lib.check(_lib.load().mifft_ragged_execute_device(self._p, _dev_ptr(src), _dev_ptr(dst),
+                                                           int(direction), int(mode),
+                                                           _stream_handle(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_p", None) and self._p.value:
+            _lib.load().mifft_ragged_plan_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PrecisionFFT:
+    """N = 4096 batched FFT in an explicit arithmetic (mifft_precision) and
+    device element type (0 complex64, 1 complex FP16) on device buffers."""
+
+    FP32, FP16_MFMA, BF16_MFMA, FP16_MFMA_SPLIT, BF16_MFMA_SPLIT = range(5)
+    C64, C32H = 0, 1
+
+    def __init__(self, size: int, precision: int, io: int = 0):
+        self._plan = ctypes.c_void_p()
+        _lib.check(_lib.load().mifft_plan_create_ex(int(size), int(precision), int(io),
+                                                    ctypes.byref(self._plan)))
+        self.size, self.precision, self.io = int(size), int(precision), int(io)
+
+    @property
+    def kernel_name(self) -> str:
+        return _lib.load().mifft_plan_kernel_name(self._plan).decode()
+
+    def execute(self, src, dst, batch: int, direction: int = FORWARD, stream=None) -> None:
+        _lib.check(_lib.load().mifft_execute_device(self._plan, _dev_ptr(src), _dev_ptr(dst),
+                                                    int(batch), int(direction),
+                                                    _stream_handle(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_plan", None) and self._plan.value:
+            _lib.load().mifft_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -25,7 +25,7 @@ __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.
 // 4x4 DFT-16 reduce to sign/swap of these).
 #define MIFFT_C16 0.92387953251128675613f   // cos(pi/8)
 #define MIFFT_S16 0.38268343236508977173f   // sin(pi/8)
-#define MIFFT_H 0.70710678118654752440f     // sqrt(2)/2
+#define MIFFT_SQRT1_2 0.70710678118654752440f     // sqrt(2)/2
 
 // -------------------------------------------------------------- butterflies
 // Forward DFT of length R on registers, natural order in and out:
@@ -52,9 +52,9 @@ __device__ __forceinline__ void dft8(float2* x) {
     dft4(e0, e1, e2, e3);
     dft4(o0, o1, o2, o3);
     // W8^1 = (h, -h), W8^2 = -i, W8^3 = (-h, -h)
-    float2 w1o = make_float2(MIFFT_H * (o1.x + o1.y), MIFFT_H * (o1.y - o1.x));
+    float2 w1o = make_float2(MIFFT_SQRT1_2 * (o1.x + o1.y), MIFFT_SQRT1_2 * (o1.y - o1.x));
     float2 w2o = mul_mi(o2);
-    float2 w3o = make_float2(MIFFT_H * (o3.y - o3.x), -MIFFT_H * (o3.x + o3.y));
+    float2 w3o = make_float2(MIFFT_SQRT1_2 * (o3.y - o3.x), -MIFFT_SQRT1_2 * (o3.x + o3.y));
     x[0] = cadd(e0, o0); x[4] = csub(e0, o0);
     x[1] = cadd(e1, w1o); x[5] = csub(e1, w1o);
     x[2] = cadd(e2, w2o); x[6] = csub(e2, w2o);
@@ -76,13 +76,13 @@ __device__ __forceinline__ void dft16(float2* x) {
     const float2 w3 = make_float2(MIFFT_S16, -MIFFT_C16);
     const float2 w9 = make_float2(-MIFFT_C16, MIFFT_S16);
     a[1][1] = cmul(a[1][1], w1);
-    a[1][2] = make_float2(MIFFT_H * (a[1][2].x + a[1][2].y), MIFFT_H * (a[1][2].y - a[1][2].x));  // W^2
+    a[1][2] = make_float2(MIFFT_SQRT1_2 * (a[1][2].x + a[1][2].y), MIFFT_SQRT1_2 * (a[1][2].y - a[1][2].x));  // W^2
     a[1][3] = cmul(a[1][3], w3);
-    a[2][1] = make_float2(MIFFT_H * (a[2][1].x + a[2][1].y), MIFFT_H * (a[2][1].y - a[2][1].x));  // W^2
+    a[2][1] = make_float2(MIFFT_SQRT1_2 * (a[2][1].x + a[2][1].y), MIFFT_SQRT1_2 * (a[2][1].y - a[2][1].x));  // W^2
     a[2][2] = mul_mi(a[2][2]);                                                                  // W^4
-    a[2][3] = make_float2(MIFFT_H * (a[2][3].y - a[2][3].x), -MIFFT_H * (a[2][3].x + a[2][3].y)); // W^6
+    a[2][3] = make_float2(MIFFT_SQRT1_2 * (a[2][3].y - a[2][3].x), -MIFFT_SQRT1_2 * (a[2][3].x + a[2][3].y)); // W^6
     a[3][1] = cmul(a[3][1], w3);
-    a[3][2] = make_float2(MIFFT_H * (a[3][2].y - a[3][2].x), -MIFFT_H * (a[3][2].x + a[3][2].y)); // W^6
+    a[3][2] = make_float2(MIFFT_SQRT1_2 * (a[3][2].y - a[3][2].x), -MIFFT_SQRT1_2 * (a[3][2].x + a[3][2].y)); // W^6
     a[3][3] = cmul(a[3][3], w9);
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) {

@@ -1,0 +1,287 @@
+// N = 4096 complex FFT with every radix-16 stage done as a GEMM on the
+// matrix cores (v_mfma_f32_16x16x32_{f16,bf16}, FP32 accumulate).
+//
+// Reference counterpart: the CT-MMA kernel (src/fft_4096_ct_mma.metal:88-206),
+// which runs radix-8 CT-DIF stages as complex 8x8 GEMMs Y = F8 * X on Apple
+// simdgroup_float8x8 tiles (Yr = Fr Xr + (-Fi) Xi, Yi = Fr Xi + Fi Xr,
+// :147-151) with a host twiddle table.  Here, for gfx950:
+//
+//  * 4096 = 16 x 16 x 16: three self-sorting Stockham passes (the same pass
+//    structure as the FP32 LDS kernels).  Pass p's 256 radix-16 butterflies
+//    are a GEMM Y[k][b] = sum_j F16[k][j] x_j[b] over 16 tiles of 16
+//    butterflies.
+//  * Complex arithmetic folded into K = 32: B holds (re, im) of x_j
+//    interleaved (K index 2j + c), A_re[k][2j] = cos, A_re[k][2j+1] = sin,
+//    A_im[k][2j] = -sin, A_im[k][2j+1] = cos (angle 2 pi j k / 16), so one
+//    16x16x32 MFMA gives Re(Y) and another Im(Y) for a 16-butterfly tile.
+//  * Stage twiddles W_4096^{j*pos*N/(16 s)} are applied in FP32 on the VALU
+//    before the operands are rounded to FP16/BF16; results stay FP32 in LDS.
+//  * hi/lo split: x = x_hi + x_lo, F = F_hi + F_lo (each rounded to the MFMA
+//    type), Y ~ F_lo x_hi + F_hi x_lo + F_hi x_hi accumulated in FP32 (three
+//    MFMAs per half-tile), recovering roughly twice the mantissa bits.
+//  * Per-transform power-of-two scaling: the input is multiplied by 2^-e
+//    (max |x| 2^-e < 1, a workgroup max-reduction), so FP16 operands stay in
+//    range through the three passes (growth <= 16 per pass); the output is
+//    multiplied back by 2^e, which is exact.
+//
+// Lane layout of v_mfma_f32_16x16x32 (checked on the device): A lane l holds
+// row l%16, K = 8*(l/16) .. +7; B lane l holds column l%16, the same K range;
+// D lane l holds column l%16, rows 4*(l/16) .. +3.  So lane (c = l%16,
+// g = l/16) supplies x_j for j = 4g .. 4g+3 of butterfly 16*tile + c and
+// receives y_k for k = 4g .. 4g+3.
+#include <hip/hip_runtime.h>
+
+#include "fft_device.hpp"
+#include "kernels.hpp"
+#include "mfma_fft.hpp"
+#include "stockham_core.hpp"
+
+namespace mifft {
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+template <bool BF>
+struct MT;
+template <>
+struct MT<false> {
+    using v8 = h8;
+    using e = _Float16;
+    static __device__ __forceinline__ f4 mma(v8 a, v8 b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MT<true> {
+    using v8 = b8;
+    using e = __bf16;
+    static __device__ __forceinline__ f4 mma(v8 a, v8 b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// complex64 or complex-FP16 element in HBM
+struct IoC64 {
+    using T = float2;
+    static __device__ __forceinline__ float2 ld(const T* p) { return *p; }
+    static __device__ __forceinline__ void st(T* p, float2 v) { *p = v; }
+};
+struct IoC32h {
+    using T = h2;
+    static __device__ __forceinline__ float2 ld(const T* p) {
+        h2 v = *p;
+        return make_float2((float)v.x, (float)v.y);
+    }
+    static __device__ __forceinline__ void st(T* p, float2 v) {
+        h2 o = {(_Float16)v.x, (_Float16)v.y};
+        *p = o;
+    }
+};
+
+// LDS index: +1 complex per 16 and +16 per 1024 spread the stride-16 and
+// stride-1024 lane groups of the MFMA operand layout over the banks.
+__device__ __forceinline__ int midx(int i) { return i + (i >> 4) + ((i >> 10) << 4); }
+constexpr int kMfmaLds = 4096 + 256 + 64;
+
+template <bool BF, bool SPLIT>
+__device__ __forceinline__ void split8(const float (&x)[8], typename MT<BF>::v8& hi,
+                                       typename MT<BF>::v8& lo) {
+    using E = typename MT<BF>::e;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        E h = (E)x[i];
+        hi[i] = h;
+        if constexpr (SPLIT) lo[i] = (E)(x[i] - (float)h);
+    }
+}
+
+template <bool BF, bool SPLIT>
+__device__ __forceinline__ f4 gemm_half(typename MT<BF>::v8 a_hi, typename MT<BF>::v8 a_lo,
+                                        typename MT<BF>::v8 b_hi, typename MT<BF>::v8 b_lo) {
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (SPLIT) {
+        acc = MT<BF>::mma(a_lo, b_hi, acc);  // small terms first
+        acc = MT<BF>::mma(a_hi, b_lo, acc);
+    }
+    return MT<BF>::mma(a_hi, b_hi, acc);
+}
+
+template <bool BF, bool SPLIT, bool INV, class IO>
+__global__ void __launch_bounds__(256)
+k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __restrict__ tw,
+               int64_t batch, float out_scale) {
+    constexpr int N = 4096;
+    __shared__ float2 lds[kMfmaLds];
+    __shared__ float red[4];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    using V8 = typename MT<BF>::v8;
+
+    // DFT-16 operands of this lane: row k = c, columns j = 4g .. 4g+3
+    V8 are_hi, are_lo, aim_hi, aim_lo;
+    {
+        float fr[8], fi[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 4 * g + i;
+            const float2 W = tw[(256 * j * c) & (N - 1)];  // exp(-2 pi i j c / 16)
+            fr[2 * i] = W.x;
+            fr[2 * i + 1] = -W.y;
+            fi[2 * i] = W.y;
+            fi[2 * i + 1] = W.x;
+        }
+        split8<BF, SPLIT>(fr, are_hi, are_lo);
+        split8<BF, SPLIT>(fi, aim_hi, aim_lo);
+    }
+
+    for (int64_t tr = blockIdx.x; tr < batch; tr += gridDim.x) {  // uniform per workgroup
+        const typename IO::T* src = in + tr * N;
+        typename IO::T* dst = out + tr * N;
+        float2 v[4][4];  // [tile][i]: x_{4g+i} of butterfly 16*(4w+tile) + c
+        float m = 0.f;
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float2 x = IO::ld(src + 16 * (4 * w + t4) + c + 256 * (4 * g + i));
+                if constexpr (INV) x.y = -x.y;
+                v[t4][i] = x;
+                m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+            }
+        // workgroup max -> power-of-two scale
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        lds_barrier();  // previous transform done with red[] and lds[]
+        if (lane == 0) red[w] = m;
+        lds_barrier();
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        int e = 0;
+        if (m > 0.f) (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+        const float sc = ldexpf(1.f, -e);
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[t4][i] = make_float2(v[t4][i].x * sc, v[t4][i].y * sc);
+
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int s = p == 0 ? 1 : (p == 1 ? 16 : 256);
+            const int tws = N / (16 * s);
+            if (p > 0) {
+#pragma unroll
+                for (int t4 = 0; t4 < 4; ++t4) {
+                    const int b = 16 * (4 * w + t4) + c;
+                    const int pos = b & (s - 1);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int j = 4 * g + i;
+                        const float2 x = lds[midx(b + 256 * j)];
+                        v[t4][i] = cmul(x, tw[j * pos * tws]);
+                    }
+                }
+            }
+            float2 y[4][4];
+#pragma unroll
+            for (int t4 = 0; t4 < 4; ++t4) {
+                float xb[8];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    xb[2 * i] = v[t4][i].x;
+                    xb[2 * i + 1] = v[t4][i].y;
+                }
+                V8 b_hi, b_lo;
+                split8<BF, SPLIT>(xb, b_hi, b_lo);
+                const f4 yr = gemm_half<BF, SPLIT>(are_hi, are_lo, b_hi, b_lo);
+                const f4 yi = gemm_half<BF, SPLIT>(aim_hi, aim_lo, b_hi, b_lo);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) y[t4][i] = make_float2(yr[i], yi[i]);
+            }
+            if (p < 2) {
+                lds_barrier();  // all reads of lds for this pass are done
+#pragma unroll
+                for (int t4 = 0; t4 < 4; ++t4) {
+                    const int b = 16 * (4 * w + t4) + c;
+                    const int pos = b & (s - 1);
+                    const int base = (b / s) * (16 * s) + pos;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) lds[midx(base + (4 * g + i) * s)] = y[t4][i];
+                }
+                lds_barrier();
+            } else {
+                // last pass: y_k of butterfly b goes to b + 256 k
+                float osc = ldexpf(out_scale, e);
+                if constexpr (INV) osc *= 1.0f / (float)N;
+#pragma unroll
+                for (int t4 = 0; t4 < 4; ++t4) {
+                    const int b = 16 * (4 * w + t4) + c;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        float2 r = make_float2(y[t4][i].x * osc, y[t4][i].y * osc);
+                        if constexpr (INV) r.y = -r.y;
+                        IO::st(dst + b + 256 * (4 * g + i), r);
+                    }
+                }
+            }
+        }
+    }
+}
+
+int device_cus() {
+    static int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 256;
+        return n > 0 ? n : 256;
+    }();
+    return cus;
+}
+
+template <bool BF, bool SPLIT, class IO>
+hipError_t launch_mfma(const void* in, void* out, const float2* tw, int64_t batch, bool inverse,
+                       float out_scale, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    // enough resident workgroups to cover the chip several times; each loops
+    const int64_t cap = (int64_t)device_cus() * 16;
+    const int64_t blocks = batch < cap ? batch : cap;
+    auto i = static_cast<const typename IO::T*>(in);
+    auto o = static_cast<typename IO::T*>(out);
+    if (inverse)
+        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, true, IO>), dim3((unsigned)blocks), dim3(256), 0,
+                           st, i, o, tw, batch, out_scale);
+    else
+        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, false, IO>), dim3((unsigned)blocks), dim3(256), 0,
+                           st, i, o, tw, batch, out_scale);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fft_mfma(int n, int precision, int io, const void* in, void* out, const float2* tw,
+                           int64_t batch, bool inverse, float out_scale, hipStream_t st) {
+    if (n != 4096) return hipErrorInvalidValue;
+    const bool c64 = io == 0;
+    switch (precision) {
+        case 1:
+            return c64 ? launch_mfma<false, false, IoC64>(in, out, tw, batch, inverse, out_scale, st)
+                       : launch_mfma<false, false, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
+        case 2:
+            return c64 ? launch_mfma<true, false, IoC64>(in, out, tw, batch, inverse, out_scale, st)
+                       : launch_mfma<true, false, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
+        case 3:
+            return c64 ? launch_mfma<false, true, IoC64>(in, out, tw, batch, inverse, out_scale, st)
+                       : launch_mfma<false, true, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
+        case 4:
+            return c64 ? launch_mfma<true, true, IoC64>(in, out, tw, batch, inverse, out_scale, st)
+                       : launch_mfma<true, true, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace mifft

@@ -1,0 +1,304 @@
+// Ragged batches: thousands of complex transforms of different power-of-two
+// lengths (64 ... 16384) in one persistent launch.
+//
+// No reference counterpart: MetalFFT.forward(batch:) rejects uneven lengths
+// (Sources/MetalFFT/FFT/MetalFFT.swift:108-114).
+//
+// Work is cut into tiles of exactly 16384 complex elements: 1 transform of
+// 16384, 2 of 8192, ..., 256 of 64 (the last tile of a length may hold
+// fewer).  A tile is one workgroup-iteration of 1024 threads: length N uses
+// N/16 threads per transform (radix-16 passes, 16 values per thread) and
+// 16384/N transforms side by side in the same 136 KiB of LDS, so every tile
+// keeps all 1024 threads and all of LDS busy whatever its length.  Tiles are
+// sorted by length, largest first, and a grid of one workgroup per CU strides
+// over them.  The same kernel launched once per length over that length's
+// tiles is the one-launch-per-size baseline.  Both forms can be captured in
+// a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ragged.hpp"
+#include "stockham_core.hpp"
+
+namespace mifft {
+
+namespace {
+
+constexpr int kTileElems = 16384;
+constexpr int kTileThreads = kTileElems / 16;
+constexpr int kTileLds = kTileElems + kTileElems / 16;  // LdsPad::len(N) * (16384 / N) for all N
+
+template <int N, bool INV>
+__device__ __noinline__ void tile_body(const float2* in, float2* out, const float2* tw,
+                                          const int64_t* offs, int count, float2* lds) {
+    constexpr int E = 16;
+    constexpr int T = N / E;
+    const int local = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const bool active = local < count;
+    const int64_t off = active ? offs[local] : 0;
+    float2* buf = lds + local * LdsPad::len(N);
+    float2 v[E];
+    load_column<N, E, INV, false>(v, in + off, t, active);
+    stockham_regs<N, E, LdsPad, 0, true>(v, buf, t, tw);
+    if (active) store_column<N, E, INV, false>(v, out + off, t);
+}
+
+// tw_all holds W_N^k for every length back to back: table N starts at N - 64.
+template <bool INV>
+__global__ void __launch_bounds__(kTileThreads)
+k_ragged(const float2* in, float2* out, const float2* __restrict__ tw_all,
+         const RaggedTile* __restrict__ tiles, const int64_t* __restrict__ offs, int tile_begin,
+         int tile_end) {
+    __shared__ float2 lds[kTileLds];
+    for (int ti = tile_begin + (int)blockIdx.x; ti < tile_end; ti += (int)gridDim.x) {
+        const RaggedTile d = tiles[ti];
+        const int64_t* o = offs + d.first;
+        switch (d.log2n) {  // uniform across the workgroup
+            case 6: tile_body<64, INV>(in, out, tw_all + 0, o, d.count, lds); break;
+            case 7: tile_body<128, INV>(in, out, tw_all + 64, o, d.count, lds); break;
+            case 8: tile_body<256, INV>(in, out, tw_all + 192, o, d.count, lds); break;
+            case 9: tile_body<512, INV>(in, out, tw_all + 448, o, d.count, lds); break;
+            case 10: tile_body<1024, INV>(in, out, tw_all + 960, o, d.count, lds); break;
+            case 11: tile_body<2048, INV>(in, out, tw_all + 1984, o, d.count, lds); break;
+            case 12: tile_body<4096, INV>(in, out, tw_all + 4032, o, d.count, lds); break;
+            case 13: tile_body<8192, INV>(in, out, tw_all + 8128, o, d.count, lds); break;
+            case 14: tile_body<16384, INV>(in, out, tw_all + 16320, o, d.count, lds); break;
+            default: break;
+        }
+    }
+}
+
+__global__ void k_twiddles_all(float2* __restrict__ tw) {
+    // table for N = 2^l occupies [N - 64, 2N - 64)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 32704) return;
+    int n = 64;
+    while (i >= 2 * n - 64) n *= 2;
+    const int k = i - (n - 64);
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)n, &s, &c);
+    tw[i] = make_float2((float)c, (float)s);
+}
+
+}  // namespace
+
+int ragged_cus() {
+    static int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 256;
+        return n > 0 ? n : 256;
+    }();
+    return cus;
+}
+
+hipError_t launch_ragged_twiddles(float2* tw_all, hipStream_t st) {
+    hipLaunchKernelGGL(k_twiddles_all, dim3((32704 + 255) / 256), dim3(256), 0, st, tw_all);
+    return hipGetLastError();
+}
+
+hipError_t launch_ragged(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tiles,
+                         const int64_t* offs, int tile_begin, int tile_end, bool inverse,
+                         hipStream_t st) {
+    const int ntiles = tile_end - tile_begin;
+    if (ntiles <= 0) return hipSuccess;
+    const int grid = std::min(ntiles, ragged_cus());
+    if (inverse)
+        hipLaunchKernelGGL(k_ragged<true>, dim3(grid), dim3(kTileThreads), 0, st, in, out, tw_all,
+                           tiles, offs, tile_begin, tile_end);
+    else
+        hipLaunchKernelGGL(k_ragged<false>, dim3(grid), dim3(kTileThreads), 0, st, in, out, tw_all,
+                           tiles, offs, tile_begin, tile_end);
+    return hipGetLastError();
+}
+
+}  // namespace mifft
+
+using namespace mifft;
+
+namespace {
+
+mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool inverse, int mode,
+                     hipStream_t st) {
+    const RaggedTile* tiles = static_cast<const RaggedTile*>(rp->d_tiles.ptr);
+    const int64_t* offs = static_cast<const int64_t*>(rp->d_offsets.ptr);
+    const float2* tw = static_cast<const float2*>(rp->d_twiddles.ptr);
+    if ((mode & MIFFT_RAGGED_PER_SIZE) == 0)
+        return check_hip(launch_ragged(in, out, tw, tiles, offs, 0, rp->ntiles, inverse, st),
+                         "ragged fft");
+    for (size_t s = 0; s + 1 < rp->size_tile_begin.size(); ++s) {
+        mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->size_tile_begin[s],
+                                                 rp->size_tile_begin[s + 1], inverse, st),
+                                   "ragged fft (per size)");
+        if (r != MIFFT_OK) return r;
+    }
+    return MIFFT_OK;
+}
+
+void drop_graph(mifft_ragged_plan* rp) {
+    if (rp->graph_exec) (void)hipGraphExecDestroy(rp->graph_exec);
+    if (rp->graph) (void)hipGraphDestroy(rp->graph);
+    rp->graph_exec = nullptr;
+    rp->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* offsets, int64_t count,
+                                      mifft_ragged_plan** out) {
+    if (!out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null plan pointer");
+    *out = nullptr;
+    if (count < 0) return fail(MIFFT_ERR_INVALID_ARGUMENT, "negative count");
+    if (count > 0 && !lengths) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null lengths");
+    std::vector<int64_t> offs(count);
+    int64_t pos = 0, extent = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        const int64_t n = lengths[i];
+        if (!supported_size(n))
+            return fail(MIFFT_ERR_UNSUPPORTED_SIZE,
+                        "Batch element " + std::to_string(i) + ": unsupported FFT size " +
+                            std::to_string(n),
+                        -1, n, i);
+        offs[i] = offsets ? offsets[i] : pos;
+        if (offs[i] < 0) return fail(MIFFT_ERR_INVALID_ARGUMENT, "negative offset");
+        pos += n;
+        extent = std::max(extent, offs[i] + n);
+    }
+    mifft_status s = context_init();
+    if (s != MIFFT_OK) return s;
+
+    // order: length descending, then input order
+    std::vector<int64_t> order(count);
+    for (int64_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return lengths[a] > lengths[b]; });
+    std::vector<int64_t> sorted_offs(count);
+    std::vector<RaggedTile> tiles;
+    std::vector<int> begins;
+    int64_t i = 0;
+    while (i < count) {
+        const int64_t n = lengths[order[i]];
+        int l = 0;
+        while ((int64_t(1) << l) < n) ++l;
+        begins.push_back((int)tiles.size());
+        const int per_tile = (int)(16384 / n);
+        int64_t j = i;
+        while (j < count && lengths[order[j]] == n) ++j;
+        for (int64_t a = i; a < j; a += per_tile) {
+            RaggedTile t;
+            t.log2n = l;
+            t.count = (int)std::min<int64_t>(per_tile, j - a);
+            t.first = a;
+            tiles.push_back(t);
+        }
+        for (int64_t a = i; a < j; ++a) sorted_offs[a] = offs[order[a]];
+        i = j;
+    }
+    begins.push_back((int)tiles.size());
+
+    mifft_ragged_plan* rp = new mifft_ragged_plan();
+    rp->count = count;
+    rp->extent = extent;
+    rp->ntiles = (int)tiles.size();
+    rp->size_tile_begin = begins;
+    for (int64_t a = 0; a < count; ++a) rp->elements += lengths[a];
+    if (hipStreamCreateWithFlags(&rp->stream, hipStreamNonBlocking) != hipSuccess) {
+        mifft_ragged_plan_destroy(rp);
+        return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
+    }
+    if ((s = rp->d_tiles.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tiles.size()))) != MIFFT_OK ||
+        (s = rp->d_offsets.reserve(sizeof(int64_t) * std::max<int64_t>(1, count))) != MIFFT_OK ||
+        (s = rp->d_twiddles.reserve(sizeof(float2) * 32704)) != MIFFT_OK) {
+        mifft_ragged_plan_destroy(rp);
+        return s;
+    }
+    if ((!tiles.empty() &&
+         (s = check_hip(hipMemcpy(rp->d_tiles.ptr, tiles.data(), sizeof(RaggedTile) * tiles.size(),
+                                  hipMemcpyHostToDevice),
+                        "hipMemcpy")) != MIFFT_OK) ||
+        (count > 0 && (s = check_hip(hipMemcpy(rp->d_offsets.ptr, sorted_offs.data(),
+                                               sizeof(int64_t) * count, hipMemcpyHostToDevice),
+                                     "hipMemcpy")) != MIFFT_OK) ||
+        (s = check_hip(launch_ragged_twiddles(static_cast<float2*>(rp->d_twiddles.ptr), rp->stream),
+                       "twiddles")) != MIFFT_OK ||
+        (s = check_hip(hipStreamSynchronize(rp->stream), "hipStreamSynchronize")) != MIFFT_OK) {
+        mifft_ragged_plan_destroy(rp);
+        return s;
+    }
+    *out = rp;
+    return MIFFT_OK;
+}
+
+mifft_status mifft_ragged_plan_destroy(mifft_ragged_plan* rp) {
+    if (!rp) return MIFFT_OK;
+    if (rp->stream) (void)hipStreamSynchronize(rp->stream);
+    drop_graph(rp);
+    rp->d_tiles.release();
+    rp->d_offsets.release();
+    rp->d_twiddles.release();
+    if (rp->stream) (void)hipStreamDestroy(rp->stream);
+    delete rp;
+    return MIFFT_OK;
+}
+
+int64_t mifft_ragged_extent(const mifft_ragged_plan* rp) { return rp ? rp->extent : -1; }
+int64_t mifft_ragged_elements(const mifft_ragged_plan* rp) { return rp ? rp->elements : -1; }
+int64_t mifft_ragged_launches(const mifft_ragged_plan* rp, int mode) {
+    if (!rp) return -1;
+    if (rp->ntiles == 0) return 0;
+    return (mode & MIFFT_RAGGED_PER_SIZE) ? (int64_t)rp->size_tile_begin.size() - 1 : 1;
+}
+
+mifft_status mifft_ragged_execute_device(mifft_ragged_plan* rp, const float* in, float* out,
+                                         mifft_direction dir, int mode, void* stream) {
+    if (!rp) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null plan");
+    if (dir != MIFFT_FORWARD && dir != MIFFT_INVERSE)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "direction must be MIFFT_FORWARD or MIFFT_INVERSE");
+    if (mode & ~(MIFFT_RAGGED_PER_SIZE | MIFFT_RAGGED_GRAPH))
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown mode bits");
+    if (rp->count == 0) return MIFFT_OK;
+    if (!in || !out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null buffer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const float2* i2 = reinterpret_cast<const float2*>(in);
+    float2* o2 = reinterpret_cast<float2*>(out);
+    const bool inverse = dir == MIFFT_INVERSE;
+    if (!(mode & MIFFT_RAGGED_GRAPH)) return enqueue(rp, i2, o2, inverse, mode, st);
+
+    // Graph form: captured once per (buffers, direction, mode), replayed after.
+    if (!rp->graph_exec || rp->g_in != in || rp->g_out != out || rp->g_dir != dir ||
+        rp->g_mode != mode) {
+        drop_graph(rp);
+        hipStream_t cap = rp->stream;
+        mifft_status s =
+            check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        if (s != MIFFT_OK) return s;
+        mifft_status se = enqueue(rp, i2, o2, inverse, mode & ~MIFFT_RAGGED_GRAPH, cap);
+        hipGraph_t g = nullptr;
+        s = check_hip(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
+        if (se != MIFFT_OK || s != MIFFT_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return se != MIFFT_OK ? se : s;
+        }
+        rp->graph = g;
+        if ((s = check_hip(hipGraphInstantiate(&rp->graph_exec, g, nullptr, nullptr, 0),
+                           "hipGraphInstantiate")) != MIFFT_OK) {
+            drop_graph(rp);
+            return s;
+        }
+        rp->g_in = in;
+        rp->g_out = out;
+        rp->g_dir = dir;
+        rp->g_mode = mode;
+    }
+    return check_hip(hipGraphLaunch(rp->graph_exec, st), "hipGraphLaunch");
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Ragged-batch plan shared by ragged.hip and the C-ABI.
+#pragma once
+#include <vector>
+
+#include "runtime.hpp"
+
+namespace mifft {
+// One workgroup-iteration: `count` transforms of length 2^log2n whose
+// offsets are offs[first .. first + count).
+struct RaggedTile {
+    int32_t log2n;
+    int32_t count;
+    int64_t first;
+};
+}  // namespace mifft
+
+struct mifft_ragged_plan {
+    int64_t count = 0;     // transforms
+    int64_t elements = 0;  // sum of lengths
+    int64_t extent = 0;    // max(offset + length): buffer length needed
+    int ntiles = 0;
+    std::vector<int> size_tile_begin;  // tile range per length, largest length first
+    mifft::DeviceBuffer d_tiles, d_offsets, d_twiddles;
+    hipStream_t stream = nullptr;  // capture stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void* g_in = nullptr;
+    const void* g_out = nullptr;
+    int g_dir = 0, g_mode = -1;
+};

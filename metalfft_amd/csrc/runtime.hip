@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "mfma_fft.hpp"
 #include "runtime.hpp"
 
 namespace mifft {
@@ -124,6 +125,10 @@ mifft_status validate_exec(const mifft_plan* plan, const void* in, const void* o
 
 mifft_status run(const mifft_plan* plan, const float* in, float* out, int64_t batch, int dir,
                  hipStream_t stream) {
+    if (plan->precision != MIFFT_PREC_FP32)
+        return check_hip(launch_fft_mfma((int)plan->n, plan->precision, plan->io, in, out,
+                                         plan->twiddles, batch, dir == MIFFT_INVERSE, 1.0f, stream),
+                         "mfma fft");
     hipError_t e = plan->variant->launch(reinterpret_cast<const float2*>(in),
                                          reinterpret_cast<float2*>(out), plan->twiddles, batch,
                                          dir == MIFFT_INVERSE, stream);
@@ -134,6 +139,8 @@ mifft_status run(const mifft_plan* plan, const float* in, float* out, int64_t ba
 mifft_status host_batch(mifft_plan* plan, const float* const* inputs, const int64_t* counts,
                         int64_t batch, float* const* outputs, int dir) {
     if (!plan) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null plan");
+    if (plan->io != MIFFT_IO_C64)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "host entry points take complex64 data");
     if (batch < 0) return fail(MIFFT_ERR_INVALID_ARGUMENT, "negative batch");
     if (batch == 0) return MIFFT_OK;
     if (!inputs || !counts || !outputs) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null array");
@@ -227,7 +234,16 @@ mifft_status mifft_device_name(char* buf, size_t len) {
 }
 
 mifft_status mifft_plan_create(int64_t n, mifft_plan** out) {
+    return mifft_plan_create_ex(n, MIFFT_PREC_FP32, MIFFT_IO_C64, out);
+}
+
+mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io io,
+                                  mifft_plan** out) {
     if (!out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null plan pointer");
+    if (precision < MIFFT_PREC_FP32 || precision > MIFFT_PREC_BF16_MFMA_SPLIT)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (io != MIFFT_IO_C64 && io != MIFFT_IO_C32H)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown io type");
     *out = nullptr;
     if (!supported_size(n))
         return fail(MIFFT_ERR_UNSUPPORTED_SIZE,
@@ -236,8 +252,16 @@ mifft_status mifft_plan_create(int64_t n, mifft_plan** out) {
                     -1, n);
     mifft_status s = context_init();
     if (s != MIFFT_OK) return s;
+    if ((precision != MIFFT_PREC_FP32 || io != MIFFT_IO_C64) && n != 4096)
+        return fail(MIFFT_ERR_KERNEL_NOT_FOUND,
+                    "HIP kernel not found: MFMA / complex-FP16 transforms are built for n = 4096");
+    if (precision == MIFFT_PREC_FP32 && io != MIFFT_IO_C64)
+        return fail(MIFFT_ERR_KERNEL_NOT_FOUND,
+                    "HIP kernel not found: complex-FP16 I/O needs an MFMA precision");
     mifft_plan* p = new mifft_plan();
     p->n = n;
+    p->precision = precision;
+    p->io = io;
     if ((s = make_stream(&p->stream)) != MIFFT_OK) {
         delete p;
         return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
@@ -274,7 +298,14 @@ mifft_status mifft_plan_destroy(mifft_plan* plan) {
 int64_t mifft_plan_size(const mifft_plan* plan) { return plan ? plan->n : -1; }
 
 const char* mifft_plan_kernel_name(const mifft_plan* plan) {
-    return plan && plan->variant ? plan->variant->name : "";
+    static const char* mfma_names[2][5] = {
+        {"", "mfma_f16_4096_c64", "mfma_bf16_4096_c64", "mfma_f16split_4096_c64",
+         "mfma_bf16split_4096_c64"},
+        {"", "mfma_f16_4096_c32h", "mfma_bf16_4096_c32h", "mfma_f16split_4096_c32h",
+         "mfma_bf16split_4096_c32h"}};
+    if (!plan) return "";
+    if (plan->precision != MIFFT_PREC_FP32) return mfma_names[plan->io][plan->precision];
+    return plan->variant ? plan->variant->name : "";
 }
 
 mifft_status mifft_execute_device(const mifft_plan* plan, const float* in, float* out, int64_t batch,

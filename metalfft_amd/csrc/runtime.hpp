@@ -20,6 +20,9 @@ mifft_status check_hip(hipError_t e, const char* what);
 
 bool supported_size(int64_t n);
 
+// Process-wide device context (thread-safe, idempotent).
+mifft_status context_init();
+
 // Device buffer that grows on demand (host-pointer entry points).
 struct DeviceBuffer {
     void* ptr = nullptr;
@@ -36,6 +39,8 @@ const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* t
 
 struct mifft_plan {
     int64_t n = 0;
+    int precision = 0;  // mifft_precision
+    int io = 0;         // mifft_io
     const mifft::StockhamVariant* variant = nullptr;
     float2* twiddles = nullptr;  // W_N^k, k < N, FP64-generated, FP32-stored
     hipStream_t stream = nullptr;

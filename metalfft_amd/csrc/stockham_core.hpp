@@ -1,0 +1,207 @@
+// LDS-resident Stockham autosort FFT templates for N = 64 ... 16384 (gfx950).
+//
+// Each transform is worked by T = N/E threads that keep E complex values in
+// registers:
+//
+//   pass 0   : v[m] = x[t + T*m] (coalesced global load) -> DFT-R     -> LDS
+//   pass p   : LDS read  v[m] = a[t + T*m] -> twiddle -> DFT-R        -> LDS
+//   last     : LDS read                     -> twiddle -> DFT-R, leaving
+//              v[m] = X[t + T*m] in registers (coalesced global store)
+//
+// Butterfly b of a radix-R pass at stride s reads x_j = a[b + j*N/R],
+// multiplies by W_N^{j*pos*N/(R*s)} (pos = b mod s), runs a DFT-R and writes
+// y_j to a'[(b/s)*R*s + pos + j*s]: the self-sorting Stockham DIT pass of the
+// reference's radix-4 and radix-8 kernels
+// (Sources/MetalFFT/Resources/fft_multisize.metal:92-111,
+//  Sources/MetalFFT/Resources/fft_4096_batched.metal:108-203), here for
+// E in {4, 8, 16}, with twiddles read from one FP32 table generated in FP64
+// instead of per-butterfly sincos.
+//
+// The inverse follows MetalFFT.inverse (Sources/MetalFFT/FFT/MetalFFT.swift:56-65):
+// conjugate on load, forward transform, conjugate and multiply by 1/N on store.
+#pragma once
+#include "fft_device.hpp"
+
+namespace mifft {
+
+// ------------------------------------------------------------- LDS layouts
+// Padded: one spare complex per 16 (N + N/16 entries).
+struct LdsPad {
+    static constexpr int len(int n) { return n + n / 16; }
+    static __device__ __forceinline__ int idx(int i) { return i + (i >> 4); }
+};
+// XOR swizzle (exactly N entries): i ^ ((i >> 4) & 31) flips bits 0-4 by
+// bits 4-8, a bijection on [0, N) for N >= 32.  Radix-16 scatter writes
+// (lane stride 16 or 256 complex) and contiguous reads then hit 32 distinct
+// 8-byte slots per 32 lanes.
+struct LdsSwz {
+    static constexpr int len(int n) { return n; }
+    static __device__ __forceinline__ int idx(int i) { return i ^ ((i >> 4) & 31); }
+};
+
+// Workgroup barrier ordering LDS only: global loads issued before it stay in
+// flight (a full __syncthreads also waits for them).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// ------------------------------------------------------- global I/O helpers
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ float2 gload(const float2* p) {
+    if constexpr (NT) {
+        f2v x = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+        return make_float2(x.x, x.y);
+    } else {
+        return *p;
+    }
+}
+
+template <bool NT>
+__device__ __forceinline__ void gstore(float2* p, float2 v) {
+    if constexpr (NT) {
+        f2v x = {v.x, v.y};
+        __builtin_nontemporal_store(x, reinterpret_cast<f2v*>(p));
+    } else {
+        *p = v;
+    }
+}
+
+// ------------------------------------------------------------ the passes
+// Runs pass P and the following ones on v (register-resident column t).
+// SYNC0: put a barrier before the first LDS write too (needed when the LDS
+// buffer was read by a previous transform in the same workgroup).
+template <int N, int E, class L, int P, bool SYNC0>
+__device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t,
+                                              const float2* __restrict__ tw) {
+    using S = Schedule<N, E>;
+    constexpr int R = S::radix(P);
+    constexpr int SD = S::stride(P);
+    constexpr int B = E / R;  // butterflies per thread in this pass
+    constexpr int T = S::T;
+    constexpr int TWS = N / (R * SD);
+    constexpr bool LAST = (P == S::passes - 1);
+
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        float2 x[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) x[j] = v[i + B * j];
+        if constexpr (SD > 1) {
+            const int pos = (t + i * T) & (SD - 1);
+#pragma unroll
+            for (int j = 1; j < R; ++j) x[j] = cmul(x[j], tw[j * pos * TWS]);
+        }
+        dft<R>(x);
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[i + B * j] = x[j];
+    }
+
+    // Last pass: R * SD == N, so y_j of butterfly b = t + i*T lands at
+    // b + j*SD = t + T*(i + B*j), i.e. v[m] already holds X[t + T*m].
+    if constexpr (!LAST) {
+        if constexpr (P > 0 || SYNC0) lds_barrier();  // this pass's reads are done
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int b = t + i * T;
+            const int pos = b & (SD - 1);
+            const int base = (b / SD) * (R * SD) + pos;
+#pragma unroll
+            for (int j = 0; j < R; ++j) buf[L::idx(base + j * SD)] = v[i + B * j];
+        }
+        lds_barrier();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = buf[L::idx(t + T * m)];
+        stockham_regs<N, E, L, P + 1, SYNC0>(v, buf, t, tw);
+    }
+}
+
+template <int N, int E, bool INV, bool NT>
+__device__ __forceinline__ void load_column(float2 (&v)[E], const float2* src, int t, bool active) {
+    constexpr int T = N / E;
+    if (active) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            float2 x = gload<NT>(src + t + T * m);
+            v[m] = INV ? conjf2(x) : x;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = make_float2(0.f, 0.f);
+    }
+}
+
+template <int N, int E, bool INV, bool NT>
+__device__ __forceinline__ void store_column(const float2 (&v)[E], float2* dst, int t) {
+    constexpr int T = N / E;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        float2 y = v[m];
+        if constexpr (INV) {
+            constexpr float inv_n = 1.0f / (float)N;
+            y = make_float2(y.x * inv_n, -y.y * inv_n);
+        }
+        gstore<NT>(dst + t + T * m, y);
+    }
+}
+
+// One launch = ceil(batch / TPW) workgroups, TPW transforms each.  `in` and
+// `out` may alias (in-place): a transform's loads all precede its stores.
+template <int N, int E, int TPW, bool INV, class L, bool NT>
+__global__ void __launch_bounds__((N / E) * TPW)
+k_stockham(const float2* in, float2* out,
+           const float2* __restrict__ tw, int64_t batch) {
+    constexpr int T = N / E;
+    __shared__ float2 lds[TPW * L::len(N)];
+    const int local = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int64_t tr = (int64_t)blockIdx.x * TPW + local;
+    const bool active = tr < batch;
+    float2* buf = lds + local * L::len(N);
+    float2 v[E];
+    load_column<N, E, INV, NT>(v, in + tr * N, t, active);
+    stockham_regs<N, E, L, 0, false>(v, buf, t, tw);
+    if (active) store_column<N, E, INV, NT>(v, out + tr * N, t);
+}
+
+// Persistent form: a grid sized to the resident workgroups loops over the
+// batch; each iteration issues the global loads of its next transform before
+// running the LDS passes of the current one, so HBM reads overlap compute.
+template <int N, int E, int TPW, bool INV, class L, bool NT>
+__global__ void __launch_bounds__((N / E) * TPW)
+k_stockham_stream(const float2* in, float2* out,
+                  const float2* __restrict__ tw, int64_t batch) {
+    constexpr int T = N / E;
+    __shared__ float2 lds[TPW * L::len(N)];
+    const int local = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    float2* buf = lds + local * L::len(N);
+    const int64_t step = (int64_t)gridDim.x * TPW;
+    float2 nxt[E];
+    int64_t tr = (int64_t)blockIdx.x * TPW + local;
+    load_column<N, E, INV, NT>(nxt, in + tr * N, t, tr < batch);
+    // the loop bound depends on blockIdx only: every thread of a workgroup
+    // runs the same iterations, so barriers stay uniform
+    for (int64_t base = (int64_t)blockIdx.x * TPW; base < batch; base += step) {
+        tr = base + local;
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        const int64_t trn = tr + step;
+        load_column<N, E, INV, NT>(nxt, in + trn * N, t, trn < batch);
+        stockham_regs<N, E, L, 0, true>(v, buf, t, tw);
+        if (tr < batch) store_column<N, E, INV, NT>(v, out + tr * N, t);
+    }
+}
+
+// Transforms per workgroup: pack small transforms up to 256 threads.
+template <int N, int E>
+constexpr int tpw_for() {
+    constexpr int T = N / E;
+    return T >= 256 ? 1 : 256 / T;
+}
+
+}  // namespace mifft

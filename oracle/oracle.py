@@ -156,3 +156,19 @@ def rel_l2(a, b) -> float:
 
 def max_abs(a, b) -> float:
     return float(np.max(np.abs(np.asarray(a, np.complex128) - np.asarray(b, np.complex128))))
+
+
+def f64_conv_grads(x: np.ndarray, taps: np.ndarray, gy: np.ndarray):
+    """FP64 gradients of y = conv(x_s, h) (per row of x) given dL/dy:
+    grad_x[s, n] = sum_k h[k] gy[s, n+k];  grad_h[k] = sum_s sum_n x[s, n] gy[s, n+k]."""
+    x = np.asarray(x, np.float64)
+    h = np.asarray(taps, np.float64)
+    gy = np.asarray(gy, np.float64)
+    S, K = x.shape[-1], h.size
+    L = gy.shape[-1]
+    m = 1 << (L + K).bit_length()
+    G = np.fft.rfft(gy, m, axis=-1)
+    # correlation c[n] = sum_k a[k] g[n + k]  <=>  irfft(G * conj(A))
+    gx = np.fft.irfft(G * np.conj(np.fft.rfft(h, m)), m, axis=-1)[..., :S]
+    gh = np.fft.irfft(G * np.conj(np.fft.rfft(x, m, axis=-1)), m, axis=-1)[..., :K]
+    return gx, gh.reshape(-1, K).sum(axis=0)

@@ -47,3 +47,18 @@ def gpu():
     _lib.load()
     m.device_name()
     return m
+
+
+def tones_c64(batch, n, seed, tones=20, noise=1e-3):
+    """Sum of `tones` complex tones per transform (random frequency and phase,
+    amplitudes log-uniform over 60 dB) plus complex N(0, noise^2) noise."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(n)
+    out = np.empty((batch, n), np.complex128)
+    for b in range(batch):
+        f = rng.uniform(0, n, tones)
+        ph = rng.uniform(0, 2 * np.pi, tones)
+        amp = 10.0 ** (-3.0 * rng.uniform(0, 1, tones))
+        out[b] = (amp[:, None] * np.exp(1j * (2 * np.pi * f[:, None] * k[None, :] / n + ph[:, None]))).sum(0)
+    out += noise * (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n)))
+    return out.astype(np.complex64)

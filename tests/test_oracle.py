@@ -165,3 +165,19 @@ def test_correlate_matches_lag_sum(oracle):
     direct = np.array([sum(pa[(i + m) % n] * pb[i] for i in range(n)) for m in range(n)])
     assert oracle.rel_l2(oracle.f64_correlate(a, b, n), direct) < 1e-12
     assert oracle.rel_l2(oracle.cpu_correlate(a, b, n), direct) < 1e-6
+
+
+def test_conv_gradients_brute_force(oracle):
+    """f64_conv_grads against the direct sums grad_x[n] = sum_k h[k] g[n+k],
+    grad_h[k] = sum_s sum_n x[s,n] g[s,n+k], and a central difference."""
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((2, 40))
+    h = rng.standard_normal(7)
+    g = rng.standard_normal((2, 46))
+    gx, gh = oracle.f64_conv_grads(x, h, g)
+    bx = np.array([[sum(h[k] * g[s, n + k] for k in range(7)) for n in range(40)] for s in range(2)])
+    bh = np.array([sum(x[s, n] * g[s, n + k] for s in range(2) for n in range(40)) for k in range(7)])
+    assert np.max(np.abs(gx - bx)) < 1e-12 and np.max(np.abs(gh - bh)) < 1e-12
+    f = lambda hv: sum(np.dot(np.convolve(x[s], hv), g[s]) for s in range(2))
+    e = np.zeros(7); e[3] = 1e-6
+    assert abs((f(h + e) - f(h - e)) / 2e-6 - gh[3]) < 1e-6

@@ -1,0 +1,135 @@
+"""Ragged batches (lengths 64 ... 16384 mixed) in one persistent launch, per
+size, and replayed from a hipGraph, against FP64 per transform."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def ragged_lengths(count, seed):
+    # log-uniform over {64, ..., 16384}
+    return (2 ** np.random.default_rng(seed).integers(6, 15, count)).astype(np.int64)
+
+
+def ragged_data(lengths, seed):
+    rng = np.random.default_rng(seed + 1000)
+    tot = int(lengths.sum())
+    return (rng.standard_normal(tot) + 1j * rng.standard_normal(tot)).astype(np.complex64)
+
+
+def check_all(oracle, lengths, x, y, inverse=False, tol=1e-6, stride=1):
+    pos = 0
+    worst = 0.0
+    for i, n in enumerate(lengths):
+        n = int(n)
+        if i % stride == 0:
+            seg = x[pos:pos + n]
+            ref = oracle.f64_ifft(seg) if inverse else oracle.f64_fft(seg)
+            worst = max(worst, oracle.rel_l2(y[pos:pos + n], ref))
+        pos += n
+    assert worst < tol, worst
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_ragged_small(gpu, oracle, mode):
+    import torch
+    lengths = ragged_lengths(300, 1)
+    x = ragged_data(lengths, 1)
+    rp = gpu.RaggedFFT(lengths)
+    assert rp.extent == x.size and rp.elements == x.size
+    assert rp.launches(mode) == (1 if mode & 1 == 0 else len(set(lengths.tolist())))
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    rp.execute(xd, yd, mode=mode)
+    torch.cuda.synchronize()
+    check_all(oracle, lengths, x, yd.cpu().numpy())
+    zd = torch.empty_like(xd)
+    rp.execute(yd, zd, direction=gpu.INVERSE, mode=mode)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(zd.cpu().numpy(), x) < 1e-6
+
+
+def test_ragged_matches_uniform_plans(gpu, oracle):
+    """Each ragged transform equals the same-length plan's output to FP32."""
+    import torch
+    lengths = np.array([64, 16384, 128, 4096, 64, 8192, 256, 512, 1024, 2048] * 3, np.int64)
+    x = ragged_data(lengths, 2)
+    rp = gpu.RaggedFFT(lengths)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    rp.execute(xd, yd)
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy()
+    pos = 0
+    for n in lengths:
+        n = int(n)
+        assert oracle.rel_l2(y[pos:pos + n], gpu.MetalFFT(n).forward(x[pos:pos + n])) < 1e-6
+        pos += n
+
+
+def test_ragged_custom_offsets_in_place(gpu, oracle):
+    import torch
+    lengths = np.array([4096, 64, 16384, 64], np.int64)
+    offsets = np.array([100, 0, 5000, 30000], np.int64)  # gaps, arbitrary order
+    rp = gpu.RaggedFFT(lengths, offsets)
+    assert rp.extent == 30064
+    buf = (np.random.default_rng(3).standard_normal(rp.extent) * (1 + 1j)).astype(np.complex64)
+    d = torch.from_numpy(buf.copy()).cuda()
+    rp.execute(d, d)  # in place
+    torch.cuda.synchronize()
+    out = d.cpu().numpy()
+    for n, o in zip(lengths, offsets):
+        assert oracle.rel_l2(out[o:o + n], oracle.f64_fft(buf[o:o + n])) < 1e-6
+    # untouched gap keeps its values
+    assert np.array_equal(out[64:100], buf[64:100])
+
+
+def test_graph_replays_match_eager(gpu, oracle):
+    """Graph replays, interleaved with eager work and syncs, all equal the
+    eager result."""
+    import torch
+    lengths = ragged_lengths(500, 4)
+    x = ragged_data(lengths, 4)
+    rp = gpu.RaggedFFT(lengths)
+    xd = torch.from_numpy(x).cuda()
+    eager = torch.empty_like(xd)
+    rp.execute(xd, eager)
+    torch.cuda.synchronize()
+    g = torch.empty_like(xd)
+    for r in range(4):
+        g.fill_(float("nan"))
+        rp.execute(xd, g, mode=gpu.RAGGED_GRAPH)
+        torch.cuda.synchronize()
+        assert torch.equal(g, eager), r
+        _ = (xd * 2).sum().item()  # unrelated eager work between replays
+
+
+def test_config2_ten_thousand(gpu, oracle):
+    """10,000 transforms, lengths log-uniform over {64..16384} (seed 1), N(0,1)
+    values: forward + inverse, persistent vs per-size; FP64 on a sample."""
+    import torch
+    lengths = ragged_lengths(10000, 1)
+    rng = np.random.default_rng(1)
+    tot = int(lengths.sum())
+    x = (rng.standard_normal(tot) + 1j * rng.standard_normal(tot)).astype(np.complex64)
+    rp = gpu.RaggedFFT(lengths)
+    xd = torch.from_numpy(x).cuda()
+    y1 = torch.empty_like(xd)
+    y2 = torch.empty_like(xd)
+    rp.execute(xd, y1, mode=gpu.RAGGED_PERSISTENT)
+    rp.execute(xd, y2, mode=gpu.RAGGED_PER_SIZE)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)  # same math, different launch structure
+    check_all(oracle, lengths, x, y1.cpu().numpy(), stride=97)
+    z = torch.empty_like(xd)
+    rp.execute(y1, z, direction=gpu.INVERSE)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(z.cpu().numpy(), x) < 1e-6
+
+
+def test_ragged_errors(gpu):
+    with pytest.raises(gpu.FFTError) as ei:
+        gpu.RaggedFFT([64, 128, 100])
+    assert ei.value.case == "unsupportedFFTSize" and ei.value.batch_index == 2
+    rp = gpu.RaggedFFT([])
+    assert rp.launches() == 0
