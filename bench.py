@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--sets", type=int, default=4, help="rotating in/out buffer sets")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--kernel", default=None, help="pin a kernel variant by name")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the MFMA / ragged / convolve / SAR measurements")
     return p.parse_args()
 
 
@@ -83,6 +85,150 @@ def cpu_baseline(n, batch, torch_seed_data):
                       f"median of 20 after 5 warmups, OpenMP over the batch",
             "one_core_value": out["one"][0], "cpu_model": model,
             "host_cpus": os.cpu_count()}
+
+
+def time_fn(fn, reps, warm, torch):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tones_gpu(batch, n, seed, torch, tones=20, noise=1e-3):
+    """Sum of 20 complex tones per transform (random frequency / phase,
+    amplitudes log-uniform over 60 dB) plus complex N(0, noise^2), on the GPU."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    f = torch.rand((batch, tones, 1), generator=g, device="cuda", dtype=torch.float64) * n
+    ph = torch.rand((batch, tones, 1), generator=g, device="cuda", dtype=torch.float64) * 2 * np.pi
+    amp = 10.0 ** (-3.0 * torch.rand((batch, tones, 1), generator=g, device="cuda",
+                                     dtype=torch.float64))
+    k = torch.arange(n, device="cuda", dtype=torch.float64)
+    x = torch.zeros((batch, n), dtype=torch.complex128, device="cuda")
+    for t in range(tones):  # one tone at a time keeps memory at batch x n
+        x += amp[:, t] * torch.exp(1j * (2 * np.pi * f[:, t] * k / n + ph[:, t]))
+    x += noise * torch.complex(torch.randn((batch, n), generator=g, device="cuda", dtype=torch.float64),
+                               torch.randn((batch, n), generator=g, device="cuda", dtype=torch.float64))
+    return x
+
+
+def extra_mfma(m, torch, O, n=4096, batch=4096):
+    """FP16 / BF16 DFT-as-GEMM forms, plain and hi/lo split, complex64 and
+    complex-FP16 device data, on the sum-of-tones workload; rel-L2 / max-abs
+    vs FP64 on 64 sampled transforms."""
+    x64 = tones_gpu(batch, n, 3, torch)
+    idx = torch.arange(0, batch, batch // 64, device="cuda")
+    ref = np.fft.fft(x64[idx].cpu().numpy(), axis=-1)
+    xc = x64.to(torch.complex64)
+    del x64
+    xh = torch.view_as_real(xc).reshape(batch, 2 * n).half().contiguous()
+    out = {"note": "4 rotating input/output sets per form (>= 512 MiB), bytes per I/O type"}
+    rot = {0: [xc] + [xc.clone() for _ in range(3)], 1: [xh] + [xh.clone() for _ in range(3)]}
+    outs = {io: [torch.empty_like(v[0]) for _ in range(4)] for io, v in rot.items()}
+
+    def timed(plan, io):
+        state = {"i": 0}
+
+        def step():
+            i = state["i"] % 4
+            plan.execute(rot[io][i], outs[io][i], batch)
+            state["i"] += 1
+        return time_fn(step, 20, 4, torch)
+
+    names = {1: "fp16", 2: "bf16", 3: "fp16_split", 4: "bf16_split"}
+    for prec, name in names.items():
+        for io in (0, 1):
+            key = f"{name}_{'c64' if io == 0 else 'c32h'}"
+            try:
+                plan = m.PrecisionFFT(n, prec, io)
+                ms = timed(plan, io)
+                plan.execute(rot[io][0], outs[io][0], batch)
+                torch.cuda.synchronize()
+                dst = outs[io][0]
+                if io == 0:
+                    y = dst[idx].cpu().numpy()
+                else:
+                    y = dst[idx].float().cpu().numpy().view(np.complex64)
+                nbytes = (16 if io == 0 else 8) * n * batch
+                out[key] = {"transforms_per_s": batch / (ms * 1e-3), "ms": ms,
+                            "gbs": nbytes / (ms * 1e-3) / 1e9, "rel_l2": O.rel_l2(y, ref),
+                            "max_abs": O.max_abs(y, ref), "kernel": plan.kernel_name}
+            except Exception as e:  # report, keep the other lines
+                out[key] = {"error": repr(e)}
+    # the FP32 path on the same data, for the same error metric
+    fft = m.MetalFFT(n)
+    state = {"i": 0}
+
+    def step32():
+        i = state["i"] % 4
+        fft.execute(rot[0][i], outs[0][i], batch)
+        state["i"] += 1
+    ms = time_fn(step32, 20, 4, torch)
+    fft.execute(rot[0][0], outs[0][0], batch)
+    torch.cuda.synchronize()
+    out["fp32_c64"] = {"transforms_per_s": batch / (ms * 1e-3), "ms": ms,
+                       "gbs": 16 * n * batch / (ms * 1e-3) / 1e9,
+                       "rel_l2": O.rel_l2(outs[0][0][idx].cpu().numpy(), ref),
+                       "kernel": fft.kernel_name}
+    return out
+
+
+def extra_ragged(m, torch, count=10000):
+    lengths = (2 ** np.random.default_rng(1).integers(6, 15, count)).astype(np.int64)
+    rp = m.RaggedFFT(lengths)
+    tot = int(lengths.sum())
+    g = torch.Generator(device="cuda").manual_seed(1)
+    x = torch.complex(torch.randn(tot, generator=g, device="cuda"),
+                      torch.randn(tot, generator=g, device="cuda"))
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    out = {"transforms": count, "elements": tot, "bytes_per_direction": 16 * tot}
+    for name, mode in (("persistent", m.RAGGED_PERSISTENT), ("per_size", m.RAGGED_PER_SIZE),
+                       ("persistent_graph", m.RAGGED_PERSISTENT | m.RAGGED_GRAPH),
+                       ("per_size_graph", m.RAGGED_PER_SIZE | m.RAGGED_GRAPH)):
+        def step():
+            rp.execute(x, y, m.FORWARD, mode)
+            rp.execute(y, z, m.INVERSE, mode)
+        ms = time_fn(step, 10, 2, torch)
+        out[name] = {"ms_fwd_inv": ms, "transforms_per_s": 2 * count / (ms * 1e-3),
+                     "gbs": 2 * 16 * tot / (ms * 1e-3) / 1e9, "launches": rp.launches(mode)}
+    return out
+
+
+def extra_convolve(m, torch, count=512, S=65536, K=1023, N=4096):
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn((count, S), generator=g, device="cuda")
+    h = torch.randn(K, generator=g, device="cuda") / np.sqrt(K)
+    cv = m.FusedConvolver(h.cpu().numpy(), N)
+    y = torch.empty((count, S + K - 1), device="cuda")
+    gy = torch.randn_like(y)
+    gx = torch.empty_like(x)
+    gh = torch.empty(K, device="cuda")
+    fwd = time_fn(lambda: cv.apply_device(x, y, S, count), 10, 2, torch)
+    bwd = time_fn(lambda: cv.backward_device(x, gy, gx, gh, S, count), 10, 2, torch)
+    fbytes = 4 * count * (S + S + K - 1)
+    bbytes = 4 * count * (S + (S + K - 1) + S)
+    return {"signals": count, "samples": S, "taps": K, "fft": N,
+            "forward_ms": fwd, "forward_gbs_alg": fbytes / (fwd * 1e-3) / 1e9,
+            "backward_ms": bwd, "backward_gbs_alg": bbytes / (bwd * 1e-3) / 1e9,
+            "signals_per_s_fwd_bwd": count / ((fwd + bwd) * 1e-3)}
+
+
+def extra_sar(m, torch, n=4096):
+    rda = m.RangeDoppler(m.RdaParams.defaults(n_range=n, n_azimuth=n, io=1))
+    g = torch.Generator(device="cuda").manual_seed(5)
+    raw = (torch.randn((n, 2 * n), generator=g, device="cuda") * 0.1).half()
+    img = torch.empty_like(raw)
+    ms = time_fn(lambda: rda.process(raw, img), 10, 2, torch)
+    nbytes = 3 * 2 * 4 * n * n  # three read+write sweeps of complex FP16
+    return {"pulses": n, "range_samples": n, "io": "complex fp16", "ms": ms,
+            "frames_per_s": 1e3 / ms, "gbs_alg": nbytes / (ms * 1e-3) / 1e9}
 
 
 def load_traffic(name):
@@ -139,6 +285,18 @@ def main():
         xh = src.cpu().numpy()
         cpu = cpu_baseline(n, batch, xh)
 
+    extras = {}
+    if not a.no_extras:
+        for key, fn in (("mfma", lambda: extra_mfma(m, torch, O)),
+                        ("ragged_10k", lambda: extra_ragged(m, torch)),
+                        ("convolve_512x65536_k1023", lambda: extra_convolve(m, torch)),
+                        ("sar_4096x4096_fp16", lambda: extra_sar(m, torch))):
+            try:
+                extras[key] = fn()
+            except Exception as e:  # keep the headline line whatever happens here
+                extras[key] = {"error": repr(e)}
+            torch.cuda.synchronize()
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -164,6 +322,7 @@ def main():
         "gflops_5nlog2n": 5 * n * np.log2(n) * value / 1e9,
         "device": m.device_name(),
         "wall_s_timed": wall,
+        "extras": extras,
     }
     print(json.dumps(line))
 

@@ -248,6 +248,61 @@ MIFFT_API mifft_status mifft_ragged_execute_device(mifft_ragged_plan *plan, cons
                                                    float *out, mifft_direction dir, int mode,
                                                    void *stream);
 
+/* -------------------------------------------- SAR range-Doppler processor */
+/* Point-target SAR focusing by the range-Doppler algorithm without RCMC:
+ * range compression (FFT, x conj(FFT(chirp)), IFFT per pulse), corner turn,
+ * azimuth compression (FFT, x H_a, IFFT per range bin).  Reference:
+ * src/radar/rda_fused_pipeline.swift:131-369, src/fft_sar_fused.metal:246-410,
+ * src/radar/rda_kernels.metal:76-91,195-238; parameters and chirp reference
+ * of src/radar/sar_simulator.swift:22-67,207-224. */
+typedef struct mifft_rda mifft_rda;
+
+typedef struct mifft_rda_params {
+    int64_t n_range;       /* range samples per pulse (FFT length 64..16384) */
+    int64_t n_azimuth;     /* pulses (FFT length 64..16384) */
+    double bandwidth;      /* chirp bandwidth, Hz */
+    double pulse_duration; /* s */
+    double sampling_rate;  /* range sampling rate, Hz */
+    double prf;            /* pulse repetition frequency, Hz */
+    double velocity;       /* platform velocity, m/s */
+    double carrier_freq;   /* Hz */
+    double center_range;   /* scene centre slant range, m */
+    int io;                /* mifft_io: MIFFT_IO_C32H (complex FP16) or MIFFT_IO_C64 */
+    int filter_convention; /* MIFFT_RDA_MATCHED or MIFFT_RDA_REFERENCE_FORMULA */
+} mifft_rda_params;
+
+/* Azimuth filter phase sign.  MATCHED: H_a = exp(+i phi), the conjugate of the
+ * echo's Doppler phase exp(-i phi) (echo phase -4 pi R / lambda,
+ * sar_simulator.swift:144), which focuses the targets.  REFERENCE_FORMULA:
+ * H_a = exp(-i phi), the literal formula of rda_kernels.metal:231-232, kept
+ * for comparison with the reference (it does not compress in azimuth). */
+#define MIFFT_RDA_MATCHED 0
+#define MIFFT_RDA_REFERENCE_FORMULA 1
+
+/* The reference's defaults (sar_simulator.swift:22-67): 4096 x 4096,
+ * B = 100 MHz, Tp = 10 us, fs = 1.2 B, PRF = 1 kHz, v = 100 m/s, fc = 10 GHz,
+ * R0 = 20 km; complex-FP16 data. */
+MIFFT_API mifft_status mifft_rda_default_params(mifft_rda_params *p);
+MIFFT_API mifft_status mifft_rda_create(const mifft_rda_params *p, mifft_rda **rda);
+MIFFT_API mifft_status mifft_rda_destroy(mifft_rda *rda);
+/* Power-of-two gains applied after range and azimuth compression (they keep
+ * complex FP16 in range: 2^-ceil(log2 chirp samples), 2^-ceil(log2 n_azimuth))
+ * and the chirp reference length in samples. */
+MIFFT_API mifft_status mifft_rda_scales(const mifft_rda *rda, double *range_scale,
+                                        double *azimuth_scale, int64_t *chirp_samples);
+/* raw: n_azimuth x n_range (pulse-major), image: n_range x n_azimuth
+ * (range-bin-major, the layout after the corner turn); raw != image.
+ * Asynchronous on `stream`; uses a plan-owned work buffer of one frame. */
+MIFFT_API mifft_status mifft_rda_process_device(mifft_rda *rda, const void *raw, void *image,
+                                                void *stream);
+/* The three steps separately (same layouts; the corner turn is out of place). */
+MIFFT_API mifft_status mifft_rda_range_compress_device(const mifft_rda *rda, const void *raw,
+                                                       void *out, void *stream);
+MIFFT_API mifft_status mifft_rda_corner_turn_device(const mifft_rda *rda, const void *in, void *out,
+                                                    void *stream);
+MIFFT_API mifft_status mifft_rda_azimuth_compress_device(const mifft_rda *rda, const void *in,
+                                                         void *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

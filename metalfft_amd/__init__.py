@@ -9,8 +9,9 @@ from .api import (MetalFFT, MultiChannelFFT, PrecisionFFT, RaggedFFT,  # noqa: F
                   RAGGED_GRAPH, RAGGED_PER_SIZE, RAGGED_PERSISTENT, SUPPORTED_SIZES,
                   device_name, kernel_variants)
 from .dsp import Convolver, Correlator, FusedConvolver, Precision  # noqa: F401
+from .sar import RangeDoppler, RdaParams  # noqa: F401
 
 __all__ = ["FFTError", "FORWARD", "INVERSE", "MetalFFT", "MultiChannelFFT",
            "SUPPORTED_SIZES", "device_name", "kernel_variants", "Convolver", "FusedConvolver",
            "Correlator", "Precision", "RaggedFFT", "PrecisionFFT", "RAGGED_GRAPH",
-           "RAGGED_PER_SIZE", "RAGGED_PERSISTENT"]
+           "RAGGED_PER_SIZE", "RAGGED_PERSISTENT", "RangeDoppler", "RdaParams"]

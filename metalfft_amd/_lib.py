@@ -94,6 +94,15 @@ _SIGNATURES = [
     ("mifft_ragged_elements", c_int64, [c_void_p]),
     ("mifft_ragged_launches", c_int64, [c_void_p, c_int]),
     ("mifft_ragged_execute_device", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    ("mifft_rda_default_params", c_int, [c_void_p]),
+    ("mifft_rda_create", c_int, [c_void_p, ctypes.POINTER(c_void_p)]),
+    ("mifft_rda_destroy", c_int, [c_void_p]),
+    ("mifft_rda_scales", c_int, [c_void_p, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_double), i64p]),
+    ("mifft_rda_process_device", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("mifft_rda_range_compress_device", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("mifft_rda_corner_turn_device", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("mifft_rda_azimuth_compress_device", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("mifft_correlator_create", c_int, [c_int64, ctypes.POINTER(c_void_p)]),
     ("mifft_correlator_destroy", c_int, [c_void_p]),
     ("mifft_correlator_fft_size", c_int64, [c_void_p]),

@@ -1,0 +1,417 @@
+// SAR range-Doppler focusing (range compression -> corner turn -> azimuth
+// FFT / matched filter / IFFT) on complex-FP16 or complex64 data.
+//
+// Reference: the fused RDA pipeline (src/radar/rda_fused_pipeline.swift:131-369)
+// with kernels fused_range_compression / fused_multiply_ifft
+// (src/fft_sar_fused.metal:246-293, :364-410), transpose_2d and
+// generate_azimuth_matched_filter (src/radar/rda_kernels.metal:76-91, :195-238)
+// and the FP16 variants (src/fft_sar_fused_fp16.metal), run as one
+// synchronous command buffer per step with device data in FP32.
+//
+// Here, three launches over data that stays in HBM (complex FP16 halves the
+// bytes of every sweep; arithmetic is FP32 in registers and LDS):
+//   1. range compression, one workgroup per pulse: FFT_Nr -> x conj(FFT(chirp))
+//      -> IFFT_Nr, times 2^-sr;
+//   2. corner turn (Na x Nr -> Nr x Na) through 64 x 64 LDS tiles, coalesced
+//      on both sides;
+//   3. azimuth compression, one workgroup per range bin: FFT_Na -> x H_a ->
+//      IFFT_Na, times 2^-sa, with
+//        H_a(f_a, R0) = exp(+i 4 pi R0 / lambda sqrt(1 - (lambda f_a / 2V)^2))
+//      (zero where the ratio reaches 0.95, as the reference), R0 the slant
+//      range of the bin: the conjugate of the echo's Doppler phase.  The
+//      reference's kernel writes exp(-i ...) (rda_kernels.metal:231-232),
+//      which equals the echo spectrum itself and leaves targets unfocused;
+//      that sign is available as MIFFT_RDA_REFERENCE_FORMULA.  The phase, ~8e6 rad at 20 km in X band, is formed in
+//      FP64 as cycles A(k) + m B D(k) (D = the square root, m = bin - Nr/2)
+//      and reduced mod 1 before the FP32 sincos; the reference evaluates it in
+//      FP32, where 8e6 rad carries ~0.5 rad of rounding.
+// The image comes out range-bin-major (Nr x Na), the layout after the corner
+// turn (the reference transposes back with a fourth naive transpose).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "convolve.hpp"
+#include "runtime.hpp"
+#include "sar.hpp"
+#include "stockham_core.hpp"
+
+namespace mifft {
+
+namespace {
+
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+template <int IO>
+struct SarIo;
+template <>
+struct SarIo<MIFFT_IO_C64> {
+    using T = float2;
+    static __device__ __forceinline__ float2 ld(const T* p) { return *p; }
+    static __device__ __forceinline__ void st(T* p, float2 v) { *p = v; }
+};
+template <>
+struct SarIo<MIFFT_IO_C32H> {
+    using T = h2;
+    static __device__ __forceinline__ float2 ld(const T* p) {
+        const h2 v = *p;
+        return make_float2((float)v.x, (float)v.y);
+    }
+    static __device__ __forceinline__ void st(T* p, float2 v) {
+        h2 o = {(_Float16)v.x, (_Float16)v.y};
+        *p = o;
+    }
+};
+
+struct RangeFilter {
+    const float2* H;  // conj(FFT(chirp reference)), n entries
+    __device__ __forceinline__ float2 at(int64_t, int k) const { return H[k]; }
+};
+
+struct AzimuthFilter {
+    const double* A;  // frac(2 R0c D(k) / lambda), cycles
+    const double* D;  // sqrt(1 - ratio^2), or -1 where the filter is zero
+    double B;         // 2 * range pixel / lambda, cycles per bin per unit D
+    int64_t half_nr;  // Nr / 2
+    float sign;       // +1: exp(+i 2 pi ph) (matched), -1: exp(-i 2 pi ph)
+    __device__ __forceinline__ float2 at(int64_t row, int k) const {
+        const double d = D[k];
+        if (d < 0.0) return make_float2(0.f, 0.f);
+        double ph = A[k] + (double)(row - half_nr) * B * d;
+        ph -= rint(ph);
+        float s, c;
+        sincospif(2.0f * (float)ph, &s, &c);
+        return make_float2(c, sign * s);
+    }
+};
+
+template <int N>
+constexpr int rows_e() { return N >= 256 ? 16 : 8; }
+template <int N>
+constexpr int rows_threads() { return (N / rows_e<N>()) * tpw_for<N, rows_e<N>()>(); }
+
+// One workgroup per row (TPW rows for small N): FFT -> x filter -> IFFT.
+template <int N, int IO, class F>
+__global__ void __launch_bounds__(rows_threads<N>())
+k_rows_compress(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out,
+                const float2* __restrict__ tw, F filt, int64_t rows, float scale) {
+    constexpr int E = N >= 256 ? 16 : 8;
+    constexpr int TPW = tpw_for<N, E>();
+    constexpr int T = N / E;
+    __shared__ float2 lds[TPW * LdsPad::len(N)];
+    const int local = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int64_t row = (int64_t)blockIdx.x * TPW + local;
+    const bool active = row < rows;
+    float2* buf = lds + local * LdsPad::len(N);
+    const typename SarIo<IO>::T* src = in + (active ? row : 0) * N;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = active ? SarIo<IO>::ld(src + t + T * m) : make_float2(0.f, 0.f);
+    stockham_regs<N, E, LdsPad, 0, false>(v, buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], filt.at(row, t + T * m)));
+    stockham_regs<N, E, LdsPad, 0, true>(v, buf, t, tw);
+    if (!active) return;
+    const float s = scale / (float)N;
+    typename SarIo<IO>::T* dst = out + row * N;
+#pragma unroll
+    for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + t + T * m, make_float2(v[m].x * s, -v[m].y * s));
+}
+
+// out[c][r] = in[r][c] for a rows x cols matrix, 64 x 64 tiles via LDS.
+template <int IO>
+__global__ void __launch_bounds__(256)
+k_corner_turn(const typename SarIo<IO>::T* __restrict__ in, typename SarIo<IO>::T* __restrict__ out,
+              int64_t rows, int64_t cols) {
+    using T = typename SarIo<IO>::T;
+    __shared__ T tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+#pragma unroll
+    for (int i = 0; i < 64; i += 4) {
+        const int64_t r = r0 + ty + i, c = c0 + tx;
+        if (r < rows && c < cols) tile[ty + i][tx] = in[r * cols + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 64; i += 4) {
+        const int64_t c = c0 + ty + i, r = r0 + tx;  // output row c, column r
+        if (r < rows && c < cols) out[c * rows + r] = tile[tx][ty + i];
+    }
+}
+
+template <int N, int IO, class F>
+hipError_t rows_n(const void* in, void* out, const float2* tw, F f, int64_t rows, float scale,
+                  hipStream_t st) {
+    constexpr int E = N >= 256 ? 16 : 8;
+    constexpr int TPW = tpw_for<N, E>();
+    const int64_t blocks = (rows + TPW - 1) / TPW;
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_rows_compress<N, IO, F>), dim3((unsigned)blocks), dim3((N / E) * TPW), 0, st,
+                       static_cast<const typename SarIo<IO>::T*>(in),
+                       static_cast<typename SarIo<IO>::T*>(out), tw, f, rows, scale);
+    return hipGetLastError();
+}
+
+template <int IO, class F>
+hipError_t rows_dispatch(int n, const void* in, void* out, const float2* tw, F f, int64_t rows,
+                         float scale, hipStream_t st) {
+    switch (n) {
+        case 64: return rows_n<64, IO, F>(in, out, tw, f, rows, scale, st);
+        case 128: return rows_n<128, IO, F>(in, out, tw, f, rows, scale, st);
+        case 256: return rows_n<256, IO, F>(in, out, tw, f, rows, scale, st);
+        case 512: return rows_n<512, IO, F>(in, out, tw, f, rows, scale, st);
+        case 1024: return rows_n<1024, IO, F>(in, out, tw, f, rows, scale, st);
+        case 2048: return rows_n<2048, IO, F>(in, out, tw, f, rows, scale, st);
+        case 4096: return rows_n<4096, IO, F>(in, out, tw, f, rows, scale, st);
+        case 8192: return rows_n<8192, IO, F>(in, out, tw, f, rows, scale, st);
+        case 16384: return rows_n<16384, IO, F>(in, out, tw, f, rows, scale, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <class F>
+hipError_t rows_io(int io, int n, const void* in, void* out, const float2* tw, F f, int64_t rows,
+                   float scale, hipStream_t st) {
+    return io == MIFFT_IO_C32H ? rows_dispatch<MIFFT_IO_C32H>(n, in, out, tw, f, rows, scale, st)
+                               : rows_dispatch<MIFFT_IO_C64>(n, in, out, tw, f, rows, scale, st);
+}
+
+hipError_t corner_turn(int io, const void* in, void* out, int64_t rows, int64_t cols, hipStream_t st) {
+    dim3 grid((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64));
+    if (io == MIFFT_IO_C32H)
+        hipLaunchKernelGGL(k_corner_turn<MIFFT_IO_C32H>, grid, dim3(256), 0, st,
+                           static_cast<const h2*>(in), static_cast<h2*>(out), rows, cols);
+    else
+        hipLaunchKernelGGL(k_corner_turn<MIFFT_IO_C64>, grid, dim3(256), 0, st,
+                           static_cast<const float2*>(in), static_cast<float2*>(out), rows, cols);
+    return hipGetLastError();
+}
+
+int ceil_log2(double x) {
+    int e = 0;
+    while (std::ldexp(1.0, e) < x) ++e;
+    return e;
+}
+
+}  // namespace
+
+mifft_status rda_range(const mifft_rda* r, const void* raw, void* out, hipStream_t st) {
+    RangeFilter f{static_cast<const float2*>(r->range_filter.ptr)};
+    return check_hip(rows_io(r->p.io, (int)r->p.n_range, raw, out, r->range_plan->twiddles, f,
+                             r->p.n_azimuth, (float)r->range_scale, st),
+                     "range compression");
+}
+
+mifft_status rda_turn(const mifft_rda* r, const void* in, void* out, hipStream_t st) {
+    return check_hip(corner_turn(r->p.io, in, out, r->p.n_azimuth, r->p.n_range, st), "corner turn");
+}
+
+mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStream_t st) {
+    AzimuthFilter f{static_cast<const double*>(r->az_a.ptr), static_cast<const double*>(r->az_d.ptr),
+                    r->az_b, r->p.n_range / 2,
+                    r->p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0f : 1.0f};
+    return check_hip(rows_io(r->p.io, (int)r->p.n_azimuth, in, out, r->az_plan->twiddles, f,
+                             r->p.n_range, (float)r->az_scale, st),
+                     "azimuth compression");
+}
+
+// Chirp reference of src/radar/sar_simulator.swift:207-224: Tp*fs samples of
+// exp(i pi Kr t^2), t centred, stored wrapped around index 0.
+std::vector<float2> chirp_reference(const mifft_rda_params& p, int64_t* samples) {
+    std::vector<float2> c((size_t)p.n_range, make_float2(0.f, 0.f));
+    const double kr = p.bandwidth / p.pulse_duration;
+    const int64_t ns = (int64_t)(p.pulse_duration * p.sampling_rate);
+    const int64_t half = ns / 2;
+    for (int64_t i = 0; i < ns; ++i) {
+        const double t = (double)(i - half) / p.sampling_rate;
+        const double ph = M_PI * kr * t * t;
+        const int64_t idx = i < half ? (p.n_range - half + i) : (i - half);
+        if (idx >= 0 && idx < p.n_range) c[(size_t)idx] = make_float2((float)std::cos(ph), (float)std::sin(ph));
+    }
+    *samples = ns;
+    return c;
+}
+
+mifft_status rda_setup(mifft_rda* r) {
+    const mifft_rda_params& p = r->p;
+    mifft_status s;
+    // range matched filter conj(FFT(chirp)) from the FP32 FFT of the chirp
+    int64_t ns = 0;
+    std::vector<float2> chirp = chirp_reference(p, &ns);
+    const size_t nb = sizeof(float2) * (size_t)p.n_range;
+    if ((s = r->range_filter.reserve(nb)) != MIFFT_OK) return s;
+    DeviceBuffer tmp;
+    if ((s = tmp.reserve(nb)) != MIFFT_OK) return s;
+    if ((s = check_hip(hipMemcpy(tmp.ptr, chirp.data(), nb, hipMemcpyHostToDevice), "hipMemcpy")) !=
+        MIFFT_OK ||
+        (s = check_hip(r->range_plan->variant->launch(static_cast<const float2*>(tmp.ptr),
+                                                      static_cast<float2*>(r->range_filter.ptr),
+                                                      r->range_plan->twiddles, 1, false, r->stream),
+                       "chirp FFT")) != MIFFT_OK ||
+        (s = check_hip(hipStreamSynchronize(r->stream), "hipStreamSynchronize")) != MIFFT_OK) {
+        tmp.release();
+        return s;
+    }
+    std::vector<float2> H((size_t)p.n_range);
+    s = check_hip(hipMemcpy(H.data(), r->range_filter.ptr, nb, hipMemcpyDeviceToHost), "hipMemcpy");
+    tmp.release();
+    if (s != MIFFT_OK) return s;
+    for (auto& h : H) h.y = -h.y;
+    if ((s = check_hip(hipMemcpy(r->range_filter.ptr, H.data(), nb, hipMemcpyHostToDevice),
+                       "hipMemcpy")) != MIFFT_OK)
+        return s;
+
+    // azimuth phase model in FP64 (rda_kernels.metal:195-238 in cycles)
+    const double c = 299792458.0;
+    const double lambda = c / p.carrier_freq;
+    const double range_pixel = c / (2.0 * p.sampling_rate);
+    std::vector<double> A((size_t)p.n_azimuth), D((size_t)p.n_azimuth);
+    for (int64_t k = 0; k < p.n_azimuth; ++k) {
+        const double fa_norm = k < p.n_azimuth / 2 ? (double)k / (double)p.n_azimuth
+                                                   : (double)(k - p.n_azimuth) / (double)p.n_azimuth;
+        const double ratio = lambda * fa_norm * p.prf / (2.0 * p.velocity);
+        if (std::fabs(ratio) < 0.95) {
+            const double d = std::sqrt(1.0 - ratio * ratio);
+            const double cyc = 2.0 * p.center_range * d / lambda;
+            D[(size_t)k] = d;
+            A[(size_t)k] = cyc - std::floor(cyc);
+        } else {
+            D[(size_t)k] = -1.0;
+            A[(size_t)k] = 0.0;
+        }
+    }
+    r->az_b = 2.0 * range_pixel / lambda;
+    const size_t ab = sizeof(double) * (size_t)p.n_azimuth;
+    if ((s = r->az_a.reserve(ab)) != MIFFT_OK || (s = r->az_d.reserve(ab)) != MIFFT_OK) return s;
+    if ((s = check_hip(hipMemcpy(r->az_a.ptr, A.data(), ab, hipMemcpyHostToDevice), "hipMemcpy")) !=
+            MIFFT_OK ||
+        (s = check_hip(hipMemcpy(r->az_d.ptr, D.data(), ab, hipMemcpyHostToDevice), "hipMemcpy")) !=
+            MIFFT_OK)
+        return s;
+    // power-of-two stage gains keep complex FP16 in range: a point target's
+    // range peak is ~(chirp samples), its azimuth peak ~(pulses) times that
+    r->chirp_samples = ns;
+    r->range_scale = std::ldexp(1.0, -ceil_log2((double)(ns > 0 ? ns : 1)));
+    r->az_scale = std::ldexp(1.0, -ceil_log2((double)p.n_azimuth));
+    return MIFFT_OK;
+}
+
+}  // namespace mifft
+
+using namespace mifft;
+
+extern "C" {
+
+mifft_status mifft_rda_default_params(mifft_rda_params* p) {
+    if (!p) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null params");
+    // src/radar/sar_simulator.swift:22-67
+    p->n_range = 4096;
+    p->n_azimuth = 4096;
+    p->bandwidth = 100e6;
+    p->pulse_duration = 10e-6;
+    p->sampling_rate = 1.2 * 100e6;
+    p->prf = 1000.0;
+    p->velocity = 100.0;
+    p->carrier_freq = 10e9;
+    p->center_range = 20000.0;
+    p->io = MIFFT_IO_C32H;
+    p->filter_convention = MIFFT_RDA_MATCHED;
+    return MIFFT_OK;
+}
+
+mifft_status mifft_rda_create(const mifft_rda_params* params, mifft_rda** out) {
+    if (!out || !params) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const mifft_rda_params& p = *params;
+    if (!supported_size(p.n_range))
+        return fail(MIFFT_ERR_UNSUPPORTED_SIZE, "Unsupported FFT size: " + std::to_string(p.n_range),
+                    -1, p.n_range);
+    if (!supported_size(p.n_azimuth))
+        return fail(MIFFT_ERR_UNSUPPORTED_SIZE, "Unsupported FFT size: " + std::to_string(p.n_azimuth),
+                    -1, p.n_azimuth);
+    if (p.io != MIFFT_IO_C32H && p.io != MIFFT_IO_C64)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown io type");
+    if (p.filter_convention != MIFFT_RDA_MATCHED && p.filter_convention != MIFFT_RDA_REFERENCE_FORMULA)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown filter convention");
+    if (!(p.bandwidth > 0 && p.pulse_duration > 0 && p.sampling_rate > 0 && p.prf > 0 &&
+          p.velocity > 0 && p.carrier_freq > 0 && p.center_range > 0))
+        return fail(MIFFT_ERR_PRECONDITION, "SAR parameters must be positive");
+    mifft_rda* r = new mifft_rda();
+    r->p = p;
+    mifft_status s;
+    if ((s = mifft_plan_create(p.n_range, &r->range_plan)) != MIFFT_OK ||
+        (s = mifft_plan_create(p.n_azimuth, &r->az_plan)) != MIFFT_OK) {
+        mifft_rda_destroy(r);
+        return s;
+    }
+    if (hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
+        mifft_rda_destroy(r);
+        return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
+    }
+    if ((s = rda_setup(r)) != MIFFT_OK) {
+        mifft_rda_destroy(r);
+        return s;
+    }
+    *out = r;
+    return MIFFT_OK;
+}
+
+mifft_status mifft_rda_destroy(mifft_rda* r) {
+    if (!r) return MIFFT_OK;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    r->range_filter.release();
+    r->az_a.release();
+    r->az_d.release();
+    r->work.release();
+    mifft_plan_destroy(r->range_plan);
+    mifft_plan_destroy(r->az_plan);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return MIFFT_OK;
+}
+
+mifft_status mifft_rda_scales(const mifft_rda* r, double* range_scale, double* azimuth_scale,
+                              int64_t* chirp_samples) {
+    if (!r) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null rda");
+    if (range_scale) *range_scale = r->range_scale;
+    if (azimuth_scale) *azimuth_scale = r->az_scale;
+    if (chirp_samples) *chirp_samples = r->chirp_samples;
+    return MIFFT_OK;
+}
+
+mifft_status mifft_rda_range_compress_device(const mifft_rda* r, const void* raw, void* out,
+                                             void* stream) {
+    if (!r || !raw || !out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
+    return rda_range(r, raw, out, static_cast<hipStream_t>(stream));
+}
+
+mifft_status mifft_rda_corner_turn_device(const mifft_rda* r, const void* in, void* out,
+                                          void* stream) {
+    if (!r || !in || !out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
+    if (in == out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "corner turn is out of place");
+    return rda_turn(r, in, out, static_cast<hipStream_t>(stream));
+}
+
+mifft_status mifft_rda_azimuth_compress_device(const mifft_rda* r, const void* in, void* out,
+                                               void* stream) {
+    if (!r || !in || !out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
+    return rda_azimuth(r, in, out, static_cast<hipStream_t>(stream));
+}
+
+mifft_status mifft_rda_process_device(mifft_rda* r, const void* raw, void* image, void* stream) {
+    if (!r || !raw || !image) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
+    if (raw == image) return fail(MIFFT_ERR_INVALID_ARGUMENT, "raw and image must differ");
+    const size_t esz = r->p.io == MIFFT_IO_C32H ? 4 : 8;
+    mifft_status s = r->work.reserve(esz * (size_t)r->p.n_range * (size_t)r->p.n_azimuth);
+    if (s != MIFFT_OK) return s;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((s = rda_range(r, raw, r->work.ptr, st)) != MIFFT_OK) return s;
+    if ((s = rda_turn(r, r->work.ptr, image, st)) != MIFFT_OK) return s;
+    return rda_azimuth(r, image, image, st);
+}
+
+}  // extern "C"

@@ -1,0 +1,16 @@
+// SAR range-Doppler processor object.
+#pragma once
+#include "runtime.hpp"
+
+struct mifft_rda {
+    mifft_rda_params p{};
+    mifft_plan* range_plan = nullptr;  // length n_range (twiddles, chirp FFT)
+    mifft_plan* az_plan = nullptr;     // length n_azimuth
+    mifft::DeviceBuffer range_filter;  // conj(FFT(chirp)), complex64
+    mifft::DeviceBuffer az_a, az_d;    // FP64 azimuth phase model per Doppler bin
+    double az_b = 0.0;                 // cycles per range bin per unit D
+    double range_scale = 1.0, az_scale = 1.0;
+    int64_t chirp_samples = 0;
+    mifft::DeviceBuffer work;          // range-compressed data (process_device)
+    hipStream_t stream = nullptr;
+};

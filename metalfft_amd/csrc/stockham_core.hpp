@@ -70,11 +70,39 @@ __device__ __forceinline__ void gstore(float2* p, float2 v) {
     }
 }
 
+// ------------------------------------------------------------ twiddles
+// w[j] = W^{j*k} for j < R from the table entries W^k, W^{2k}, W^{4k}, W^{8k}
+// (R/4.. loads instead of R - 1); every other power is a product of at most
+// three table entries, so it carries at most three FP32 roundings beyond the
+// table's own (~1e-7 relative).
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2 (&w)[R], const float2* __restrict__ tw, int k) {
+    w[0] = make_float2(1.f, 0.f);
+    if constexpr (R >= 2) w[1] = tw[k];
+    if constexpr (R >= 4) {
+        w[2] = tw[2 * k];
+        w[3] = cmul(w[1], w[2]);
+    }
+    if constexpr (R >= 8) {
+        w[4] = tw[4 * k];
+        w[5] = cmul(w[1], w[4]);
+        w[6] = cmul(w[2], w[4]);
+        w[7] = cmul(w[3], w[4]);
+    }
+    if constexpr (R >= 16) {
+        w[8] = tw[8 * k];
+#pragma unroll
+        for (int j = 9; j < 16; ++j) w[j] = cmul(w[j - 8], w[8]);
+    }
+}
+
 // ------------------------------------------------------------ the passes
 // Runs pass P and the following ones on v (register-resident column t).
 // SYNC0: put a barrier before the first LDS write too (needed when the LDS
 // buffer was read by a previous transform in the same workgroup).
-template <int N, int E, class L, int P, bool SYNC0>
+// TW4: derive twiddle powers from four table loads (twiddle_powers) instead
+// of loading all R - 1.
+template <int N, int E, class L, int P, bool SYNC0, bool TW4 = false>
 __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t,
                                               const float2* __restrict__ tw) {
     using S = Schedule<N, E>;
@@ -92,8 +120,15 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
         for (int j = 0; j < R; ++j) x[j] = v[i + B * j];
         if constexpr (SD > 1) {
             const int pos = (t + i * T) & (SD - 1);
+            if constexpr (TW4) {
+                float2 w[R];
+                twiddle_powers<R>(w, tw, pos * TWS);
 #pragma unroll
-            for (int j = 1; j < R; ++j) x[j] = cmul(x[j], tw[j * pos * TWS]);
+                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
+            } else {
+#pragma unroll
+                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], tw[j * pos * TWS]);
+            }
         }
         dft<R>(x);
 #pragma unroll
@@ -115,7 +150,7 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
         lds_barrier();
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[L::idx(t + T * m)];
-        stockham_regs<N, E, L, P + 1, SYNC0>(v, buf, t, tw);
+        stockham_regs<N, E, L, P + 1, SYNC0, TW4>(v, buf, t, tw);
     }
 }
 
@@ -150,7 +185,7 @@ __device__ __forceinline__ void store_column(const float2 (&v)[E], float2* dst, 
 
 // One launch = ceil(batch / TPW) workgroups, TPW transforms each.  `in` and
 // `out` may alias (in-place): a transform's loads all precede its stores.
-template <int N, int E, int TPW, bool INV, class L, bool NT>
+template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = false>
 __global__ void __launch_bounds__((N / E) * TPW)
 k_stockham(const float2* in, float2* out,
            const float2* __restrict__ tw, int64_t batch) {
@@ -163,15 +198,17 @@ k_stockham(const float2* in, float2* out,
     float2* buf = lds + local * L::len(N);
     float2 v[E];
     load_column<N, E, INV, NT>(v, in + tr * N, t, active);
-    stockham_regs<N, E, L, 0, false>(v, buf, t, tw);
+    stockham_regs<N, E, L, 0, false, TW4>(v, buf, t, tw);
     if (active) store_column<N, E, INV, NT>(v, out + tr * N, t);
 }
 
 // Persistent form: a grid sized to the resident workgroups loops over the
 // batch; each iteration issues the global loads of its next transform before
 // running the LDS passes of the current one, so HBM reads overlap compute.
-template <int N, int E, int TPW, bool INV, class L, bool NT>
+// WPE: minimum waves per SIMD the register allocation must allow (0 = free).
+template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = false, int WPE = 0>
 __global__ void __launch_bounds__((N / E) * TPW)
+__attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k_stockham_stream(const float2* in, float2* out,
                   const float2* __restrict__ tw, int64_t batch) {
     constexpr int T = N / E;
@@ -192,7 +229,7 @@ k_stockham_stream(const float2* in, float2* out,
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         const int64_t trn = tr + step;
         load_column<N, E, INV, NT>(nxt, in + trn * N, t, trn < batch);
-        stockham_regs<N, E, L, 0, true>(v, buf, t, tw);
+        stockham_regs<N, E, L, 0, true, TW4>(v, buf, t, tw);
         if (tr < batch) store_column<N, E, INV, NT>(v, out + tr * N, t);
     }
 }

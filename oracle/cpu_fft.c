@@ -364,3 +364,52 @@ int oracle_num_threads(void) {
     return 1;
 #endif
 }
+
+/* Raw SAR echo of point targets, following src/radar/sar_simulator.swift:93-166
+ * (flat-earth broadside geometry): for pulse az at slow time
+ * eta = -(Na/2)/PRF + az/PRF and range sample rg at fast time
+ * tau = 2 Rc/c - (Nr/2)/fs + rg/fs, each target (R0, x_t, a) adds
+ * a * exp(i (pi Kr dt^2 - 4 pi R(eta)/lambda)) when |dt| <= Tp/2, with
+ * R(eta) = sqrt(R0^2 + (v eta - x_t)^2), dt = tau - 2 R(eta)/c.  The target
+ * amplitude a is complex (the reference uses real amplitudes).
+ * prm = {B, Tp, fs, PRF, v, fc, Rc}; targets = {R0, x_t, re(a), im(a)} each;
+ * out: complex128, Na x Nr, accumulated (caller zeroes it). */
+void oracle_sar_simulate(int64_t nr, int64_t na, const double *prm, int ntargets,
+                         const double *targets, double *out, int threads) {
+    const double c = 299792458.0;
+    const double B = prm[0], Tp = prm[1], fs = prm[2], prf = prm[3], v = prm[4], fc = prm[5],
+                 rc = prm[6];
+    const double kr = B / Tp, lambda = c / fc;
+    const double t0 = rc * 2.0 / c - (double)(nr / 2) / fs;
+    const double eta0 = -(double)(na / 2) / prf;
+    (void)threads;
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(static) num_threads(threads)
+#endif
+    for (int64_t az = 0; az < na; ++az) {
+        const double eta = eta0 + (double)az / prf;
+        for (int k = 0; k < ntargets; ++k) {
+            const double r0 = targets[4 * k], xt = targets[4 * k + 1];
+            const double ar = targets[4 * k + 2], ai = targets[4 * k + 3];
+            const double dx = v * eta - xt;
+            const double reta = sqrt(r0 * r0 + dx * dx);
+            const double tau0 = 2.0 * reta / c;
+            /* samples with |tau - tau0| <= Tp/2 */
+            int64_t lo = (int64_t)ceil((tau0 - Tp / 2.0 - t0) * fs) - 1;
+            int64_t hi = (int64_t)floor((tau0 + Tp / 2.0 - t0) * fs) + 1;
+            if (lo < 0) lo = 0;
+            if (hi > nr - 1) hi = nr - 1;
+            const double rphase = -4.0 * M_PI * reta / lambda;
+            for (int64_t rg = lo; rg <= hi; ++rg) {
+                const double dt = t0 + (double)rg / fs - tau0;
+                if (fabs(dt) > Tp / 2.0) continue;
+                const double ph = M_PI * kr * dt * dt + rphase;
+                const double cr = cos(ph), ci = sin(ph);
+                double *o = out + 2 * (az * nr + rg);
+                o[0] += ar * cr - ai * ci;
+                o[1] += ar * ci + ai * cr;
+            }
+        }
+    }
+}

@@ -172,3 +172,98 @@ def f64_conv_grads(x: np.ndarray, taps: np.ndarray, gy: np.ndarray):
     gx = np.fft.irfft(G * np.conj(np.fft.rfft(h, m)), m, axis=-1)[..., :S]
     gh = np.fft.irfft(G * np.conj(np.fft.rfft(x, m, axis=-1)), m, axis=-1)[..., :K]
     return gx, gh.reshape(-1, K).sum(axis=0)
+
+
+# ---------------------------------------------------------------- SAR (FP64)
+C_LIGHT = 299792458.0
+
+
+def sar_default_params():
+    """src/radar/sar_simulator.swift:22-67 (4096 x 4096)."""
+    return dict(n_range=4096, n_azimuth=4096, bandwidth=100e6, pulse_duration=10e-6,
+                sampling_rate=120e6, prf=1000.0, velocity=100.0, carrier_freq=10e9,
+                center_range=20000.0)
+
+
+def sar_simulate(p, targets, threads=8):
+    """Raw echo (complex128, Na x Nr) of targets [(R0, x_t, complex amp)]."""
+    L = lib()
+    L.oracle_sar_simulate.argtypes = [ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.oracle_sar_simulate.restype = None
+    prm = np.array([p["bandwidth"], p["pulse_duration"], p["sampling_rate"], p["prf"],
+                    p["velocity"], p["carrier_freq"], p["center_range"]], np.float64)
+    t = np.array([[r, x, a.real, a.imag] for r, x, a in targets], np.float64).reshape(-1)
+    out = np.zeros((p["n_azimuth"], p["n_range"]), np.complex128)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    L.oracle_sar_simulate(p["n_range"], p["n_azimuth"], dp(prm), len(targets), dp(t),
+                          dp(out.view(np.float64)), threads)
+    return out
+
+
+def sar_chirp_reference(p):
+    """src/radar/sar_simulator.swift:207-224 in FP64."""
+    nr = p["n_range"]
+    kr = p["bandwidth"] / p["pulse_duration"]
+    ns = int(p["pulse_duration"] * p["sampling_rate"])
+    half = ns // 2
+    c = np.zeros(nr, np.complex128)
+    i = np.arange(ns)
+    t = (i - half) / p["sampling_rate"]
+    idx = np.where(i < half, nr - half + i, i - half)
+    ok = (idx >= 0) & (idx < nr)
+    c[idx[ok]] = np.exp(1j * np.pi * kr * t[ok] ** 2)
+    return c, ns
+
+
+def sar_azimuth_filter(p, reference_formula=False):
+    """H_a (Nr x Na) of src/radar/rda_kernels.metal:195-238 in FP64:
+    exp(+i phi) (matched, conjugate of the echo's Doppler phase) or, with
+    reference_formula, the kernel's literal exp(-i phi)."""
+    nr, na = p["n_range"], p["n_azimuth"]
+    lam = C_LIGHT / p["carrier_freq"]
+    pixel = C_LIGHT / (2.0 * p["sampling_rate"])
+    r0 = p["center_range"] + (np.arange(nr) - nr // 2) * pixel
+    k = np.arange(na)
+    fa = np.where(k < na // 2, k, k - na) / na * p["prf"]
+    ratio = lam * fa / (2.0 * p["velocity"])
+    d = np.sqrt(np.maximum(0.0, 1.0 - ratio ** 2))
+    cyc = 2.0 * r0[:, None] * d[None, :] / lam
+    cyc -= np.round(cyc)
+    h = np.exp((2j if not reference_formula else -2j) * np.pi * cyc)
+    h[:, np.abs(ratio) >= 0.95] = 0
+    return h
+
+
+def f64_rda(raw, p, range_scale=1.0, azimuth_scale=1.0, reference_formula=False):
+    """Range compression -> corner turn -> azimuth compression in FP64;
+    image is range-bin-major (Nr x Na)."""
+    chirp, _ = sar_chirp_reference(p)
+    hr = np.conj(np.fft.fft(chirp))
+    rc = np.fft.ifft(np.fft.fft(np.asarray(raw, np.complex128), axis=1) * hr, axis=1) * range_scale
+    ct = np.ascontiguousarray(rc.T)
+    del rc
+    ha = sar_azimuth_filter(p, reference_formula)
+    return np.fft.ifft(np.fft.fft(ct, axis=1) * ha, axis=1) * azimuth_scale
+
+
+def sar_targets(p, count, seed):
+    """`count` targets at random range / azimuth inside the swath, unit-modulus
+    random phase (the synthetic SAR workload)."""
+    rng = np.random.default_rng(seed)
+    r0 = p["center_range"] + rng.uniform(-1000.0, 1000.0, count)
+    span = 0.7 * (p["n_azimuth"] / 2) / p["prf"] * p["velocity"]
+    xt = rng.uniform(-span, span, count)
+    amp = np.exp(1j * rng.uniform(0, 2 * np.pi, count))
+    return list(zip(r0, xt, amp))
+
+
+def sar_expected_bins(p, r0, xt):
+    """Pixel of a focused target (sar_simulator.swift:112-113): range bin,
+    azimuth bin."""
+    t0 = p["center_range"] * 2.0 / C_LIGHT - (p["n_range"] // 2) / p["sampling_rate"]
+    rb = (2.0 * r0 / C_LIGHT - t0) * p["sampling_rate"]
+    ab = (xt / p["velocity"] + (p["n_azimuth"] // 2) / p["prf"]) * p["prf"]
+    return rb, ab
