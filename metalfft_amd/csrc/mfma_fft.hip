@@ -19,6 +19,9 @@
 //  * hi/lo split: x = x_hi + x_lo, F = F_hi + F_lo (each rounded to the MFMA
 //    type), Y ~ F_lo x_hi + F_hi x_lo + F_hi x_hi accumulated in FP32 (three
 //    MFMAs per half-tile), recovering roughly twice the mantissa bits.
+//  * Persistent workgroups: the next transform streams HBM -> LDS by DMA
+//    (global_load_lds) while the current one computes; the stage twiddles
+//    are loaded into registers once per workgroup.
 //  * Per-transform power-of-two scaling: the input is multiplied by 2^-e
 //    (max |x| 2^-e < 1, a workgroup max-reduction), so FP16 operands stay in
 //    range through the three passes (growth <= 16 per pass); the output is
@@ -115,6 +118,9 @@ __global__ void __launch_bounds__(256)
 k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __restrict__ tw,
                int64_t batch, float out_scale) {
     constexpr int N = 4096;
+    // stage: the next transform as it sits in HBM (DMA target, natural order);
+    // lds: FP32 exchange buffer of the passes.
+    __shared__ typename IO::T stage[N];
     __shared__ float2 lds[kMfmaLds];
     __shared__ float red[4];
     const int lane = threadIdx.x & 63;
@@ -139,17 +145,37 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         split8<BF, SPLIT>(fr, are_hi, are_lo);
         split8<BF, SPLIT>(fi, aim_hi, aim_lo);
     }
+    // stage twiddles of passes 1 (s = 16) and 2 (s = 256): W_4096^{j*pos*N/(16 s)}
+    float2 twr[2][4][4];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int s = q == 0 ? 16 : 256;
+        const int tws = N / (16 * s);
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4) {
+            const int pos = (16 * (4 * w + t4) + c) & (s - 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) twr[q][t4][i] = tw[(4 * g + i) * pos * tws];
+        }
+    }
 
-    for (int64_t tr = blockIdx.x; tr < batch; tr += gridDim.x) {  // uniform per workgroup
-        const typename IO::T* src = in + tr * N;
+    constexpr int kElemBytes = sizeof(typename IO::T);
+    constexpr int kCount = N * kElemBytes / 8;  // transform size in float2 units (DMA granule)
+    int64_t tr = blockIdx.x;
+    if (tr < batch)
+        dma_transform(reinterpret_cast<float2*>(stage),
+                      reinterpret_cast<const float2*>(in + tr * N), kCount, threadIdx.x, 256);
+    for (; tr < batch; tr += gridDim.x) {  // uniform per workgroup
         typename IO::T* dst = out + tr * N;
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA (and earlier stores) done
+        ds_barrier();                   // every wave's DMA landed; previous transform done
         float2 v[4][4];  // [tile][i]: x_{4g+i} of butterfly 16*(4w+tile) + c
         float m = 0.f;
 #pragma unroll
         for (int t4 = 0; t4 < 4; ++t4)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float2 x = IO::ld(src + 16 * (4 * w + t4) + c + 256 * (4 * g + i));
+                float2 x = IO::ld(stage + 16 * (4 * w + t4) + c + 256 * (4 * g + i));
                 if constexpr (INV) x.y = -x.y;
                 v[t4][i] = x;
                 m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
@@ -157,10 +183,13 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         // workgroup max -> power-of-two scale
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        lds_barrier();  // previous transform done with red[] and lds[]
         if (lane == 0) red[w] = m;
-        lds_barrier();
+        ds_barrier();  // stage fully read, red[] written
         m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        if (tr + gridDim.x < batch)
+            dma_transform(reinterpret_cast<float2*>(stage),
+                          reinterpret_cast<const float2*>(in + (tr + gridDim.x) * N), kCount,
+                          threadIdx.x, 256);
         int e = 0;
         if (m > 0.f) (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
         const float sc = ldexpf(1.f, -e);
@@ -172,17 +201,14 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
             const int s = p == 0 ? 1 : (p == 1 ? 16 : 256);
-            const int tws = N / (16 * s);
             if (p > 0) {
 #pragma unroll
                 for (int t4 = 0; t4 < 4; ++t4) {
                     const int b = 16 * (4 * w + t4) + c;
-                    const int pos = b & (s - 1);
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int j = 4 * g + i;
-                        const float2 x = lds[midx(b + 256 * j)];
-                        v[t4][i] = cmul(x, tw[j * pos * tws]);
+                        v[t4][i] = cmul(lds[midx(b + 256 * j)], twr[p - 1][t4][i]);
                     }
                 }
             }
@@ -203,7 +229,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
                 for (int i = 0; i < 4; ++i) y[t4][i] = make_float2(yr[i], yi[i]);
             }
             if (p < 2) {
-                lds_barrier();  // all reads of lds for this pass are done
+                if (p > 0) ds_barrier();  // all reads of lds for this pass are done
 #pragma unroll
                 for (int t4 = 0; t4 < 4; ++t4) {
                     const int b = 16 * (4 * w + t4) + c;
@@ -212,7 +238,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
 #pragma unroll
                     for (int i = 0; i < 4; ++i) lds[midx(base + (4 * g + i) * s)] = y[t4][i];
                 }
-                lds_barrier();
+                ds_barrier();
             } else {
                 // last pass: y_k of butterfly b goes to b + 256 k
                 float osc = ldexpf(out_scale, e);
@@ -247,8 +273,16 @@ template <bool BF, bool SPLIT, class IO>
 hipError_t launch_mfma(const void* in, void* out, const float2* tw, int64_t batch, bool inverse,
                        float out_scale, hipStream_t st) {
     if (batch <= 0) return hipSuccess;
-    // enough resident workgroups to cover the chip several times; each loops
-    const int64_t cap = (int64_t)device_cus() * 16;
+    // resident workgroups only (each loops over transforms, prefetching)
+    static int per_cu = [] {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void*>(&k_fft4096_mfma<BF, SPLIT, false, IO>), 256, 0) !=
+                hipSuccess || nb <= 0)
+            nb = 1;
+        return nb;
+    }();
+    const int64_t cap = (int64_t)device_cus() * per_cu;
     const int64_t blocks = batch < cap ? batch : cap;
     auto i = static_cast<const typename IO::T*>(in);
     auto o = static_cast<typename IO::T*>(out);

@@ -2,12 +2,17 @@
 //
 // Variants per length N: E in {4, 8, 16} complex values per thread (radix-4
 // family as in Sources/MetalFFT/Resources/fft_multisize.metal, radix-8 as in
-// Sources/MetalFFT/Resources/fft_4096_batched.metal, and radix-16), each in
-// two forms:
-//   one-shot   : ceil(batch / TPW) workgroups, padded LDS layout;
-//   persistent : grid = resident workgroups, prefetching the next transform's
-//                HBM reads during the LDS passes, XOR-swizzled LDS (exactly
-//                8*N bytes per transform) and non-temporal global I/O.
+// Sources/MetalFFT/Resources/fft_4096_batched.metal, and radix-16), in the
+// forms:
+//   one-shot      : ceil(batch / TPW) workgroups, padded LDS layout;
+//   _swz_nt       : one-shot, XOR-swizzled LDS (exactly 8*N bytes per
+//                   transform), non-temporal global I/O;
+//   _stream       : persistent grid prefetching the next transform into
+//                   registers during the LDS passes;
+//   _dma          : persistent grid, next transforms copied HBM -> LDS by
+//                   global_load_lds DMA into the other half of a double
+//                   buffer, twiddles held in registers (N >= 128).
+// All read twiddles as W^k, W^2k, W^4k, W^8k and derive the other powers.
 // The autotuner picks one per size (autotune.hip).
 #include "kernels.hpp"
 #include "stockham_core.hpp"
@@ -26,6 +31,33 @@ int device_cu_count() {
     return cus;
 }
 }  // namespace
+
+template <int N, int E>
+static hipError_t launch_dma(const float2* in, float2* out, const float2* tw, int64_t batch,
+                             bool inverse, hipStream_t stream) {
+    constexpr int TPW = tpw_for<N, E>();
+    constexpr int threads = (N / E) * TPW;
+    static_assert(2 * TPW * N * 8 <= 160 * 1024, "LDS over 160 KiB");
+    if (batch <= 0) return hipSuccess;
+    static int per_cu = [] {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, reinterpret_cast<const void*>(&k_stockham_dma<N, E, TPW, false>), threads, 0) !=
+                hipSuccess || nb <= 0)
+            nb = 1;
+        return nb;
+    }();
+    int64_t blocks = (batch + TPW - 1) / TPW;
+    const int64_t resident = (int64_t)per_cu * device_cu_count();
+    if (blocks > resident) blocks = resident;
+    if (inverse)
+        hipLaunchKernelGGL((k_stockham_dma<N, E, TPW, true>), dim3((unsigned)blocks), dim3(threads), 0,
+                           stream, in, out, tw, batch);
+    else
+        hipLaunchKernelGGL((k_stockham_dma<N, E, TPW, false>), dim3((unsigned)blocks), dim3(threads), 0,
+                           stream, in, out, tw, batch);
+    return hipGetLastError();
+}
 
 template <int N, int E, class L, bool NT, bool PERS>
 static hipError_t launch_stockham(const float2* in, float2* out, const float2* tw, int64_t batch,
@@ -68,23 +100,38 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
 
 #define MIFFT_V(N, E) \
     {N, E, tpw_for<N, E>(), &launch_stockham<N, E, LdsPad, false, false>, "stockham_r" #E "_" #N}
+#define MIFFT_VS(N, E)                                                                          \
+    {N, E, tpw_for<N, E>(), &launch_stockham<N, E, LdsSwz, true, false>,                        \
+     "stockham_r" #E "_" #N "_swz_nt"}
 #define MIFFT_VP(N, E)                                                                          \
     {N, E, tpw_for<N, E>(), &launch_stockham<N, E, LdsSwz, true, true>,                         \
      "stockham_r" #E "_" #N "_stream"}
+#define MIFFT_VD(N, E) \
+    {N, E, tpw_for<N, E>(), &launch_dma<N, E>, "stockham_r" #E "_" #N "_dma"}
 
 static const StockhamVariant kVariants[] = {
-    MIFFT_V(64, 4),    MIFFT_V(64, 8),    MIFFT_V(64, 16),    MIFFT_VP(64, 8),    MIFFT_VP(64, 16),
-    MIFFT_V(128, 4),   MIFFT_V(128, 8),   MIFFT_V(128, 16),   MIFFT_VP(128, 8),   MIFFT_VP(128, 16),
-    MIFFT_V(256, 4),   MIFFT_V(256, 8),   MIFFT_V(256, 16),   MIFFT_VP(256, 8),   MIFFT_VP(256, 16),
-    MIFFT_V(512, 4),   MIFFT_V(512, 8),   MIFFT_V(512, 16),   MIFFT_VP(512, 8),   MIFFT_VP(512, 16),
-    MIFFT_V(1024, 4),  MIFFT_V(1024, 8),  MIFFT_V(1024, 16),  MIFFT_VP(1024, 8),  MIFFT_VP(1024, 16),
-    MIFFT_V(2048, 4),  MIFFT_V(2048, 8),  MIFFT_V(2048, 16),  MIFFT_VP(2048, 8),  MIFFT_VP(2048, 16),
-    MIFFT_V(4096, 4),  MIFFT_V(4096, 8),  MIFFT_V(4096, 16),  MIFFT_VP(4096, 8),  MIFFT_VP(4096, 16),
-    MIFFT_V(8192, 8),  MIFFT_V(8192, 16), MIFFT_VP(8192, 8),  MIFFT_VP(8192, 16),
-    MIFFT_V(16384, 16), MIFFT_VP(16384, 16),
+    MIFFT_V(64, 4),    MIFFT_V(64, 8),    MIFFT_V(64, 16),    MIFFT_VS(64, 8),    MIFFT_VP(64, 8),
+    MIFFT_VP(64, 16),
+    MIFFT_V(128, 4),   MIFFT_V(128, 8),   MIFFT_V(128, 16),   MIFFT_VS(128, 16),  MIFFT_VP(128, 8),
+    MIFFT_VP(128, 16), MIFFT_VD(128, 8),  MIFFT_VD(128, 16),
+    MIFFT_V(256, 4),   MIFFT_V(256, 8),   MIFFT_V(256, 16),   MIFFT_VS(256, 16),  MIFFT_VP(256, 8),
+    MIFFT_VP(256, 16), MIFFT_VD(256, 8),  MIFFT_VD(256, 16),
+    MIFFT_V(512, 4),   MIFFT_V(512, 8),   MIFFT_V(512, 16),   MIFFT_VS(512, 16),  MIFFT_VP(512, 8),
+    MIFFT_VP(512, 16), MIFFT_VD(512, 8),  MIFFT_VD(512, 16),
+    MIFFT_V(1024, 4),  MIFFT_V(1024, 8),  MIFFT_V(1024, 16),  MIFFT_VS(1024, 16), MIFFT_VP(1024, 8),
+    MIFFT_VP(1024, 16), MIFFT_VD(1024, 8), MIFFT_VD(1024, 16),
+    MIFFT_V(2048, 4),  MIFFT_V(2048, 8),  MIFFT_V(2048, 16),  MIFFT_VS(2048, 16), MIFFT_VP(2048, 8),
+    MIFFT_VP(2048, 16), MIFFT_VD(2048, 8), MIFFT_VD(2048, 16),
+    MIFFT_V(4096, 4),  MIFFT_V(4096, 8),  MIFFT_V(4096, 16),  MIFFT_VS(4096, 16), MIFFT_VP(4096, 8),
+    MIFFT_VP(4096, 16), MIFFT_VD(4096, 8), MIFFT_VD(4096, 16),
+    MIFFT_V(8192, 8),  MIFFT_V(8192, 16), MIFFT_VS(8192, 16), MIFFT_VP(8192, 8),  MIFFT_VP(8192, 16),
+    MIFFT_VD(8192, 8), MIFFT_VD(8192, 16),
+    MIFFT_V(16384, 16), MIFFT_VS(16384, 16), MIFFT_VP(16384, 16),
 };
 #undef MIFFT_V
+#undef MIFFT_VS
 #undef MIFFT_VP
+#undef MIFFT_VD
 
 const StockhamVariant* stockham_variants(int* count) {
     *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));

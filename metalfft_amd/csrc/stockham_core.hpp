@@ -47,6 +47,16 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Barrier that orders this workgroup's DS (LDS) instructions only: each wave
+// drains its own LDS queue (lgkmcnt) and meets the others at s_barrier; the
+// asm memory clobber keeps the compiler from moving memory accesses across.
+// Unlike the fence form it does not wait for LDS-DMA transfers in flight
+// (they count in vmcnt), which is the point in the DMA streaming kernel,
+// where each DMA target is waited for explicitly before it is read.
+__device__ __forceinline__ void ds_barrier() {
+    __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ------------------------------------------------------- global I/O helpers
 typedef float f2v __attribute__((ext_vector_type(2)));
 
@@ -102,9 +112,13 @@ __device__ __forceinline__ void twiddle_powers(float2 (&w)[R], const float2* __r
 // buffer was read by a previous transform in the same workgroup).
 // TW4: derive twiddle powers from four table loads (twiddle_powers) instead
 // of loading all R - 1.
-template <int N, int E, class L, int P, bool SYNC0, bool TW4 = false>
+// PRE: twiddles of passes >= 1 come from `pre` (registers, PreTwiddles).
+// DSB: use ds_barrier (LDS-only ordering) instead of the fence barrier.
+template <int N, int E, class L, int P, bool SYNC0, bool TW4 = true, bool DSB = false,
+          bool PRE = false>
 __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t,
-                                              const float2* __restrict__ tw) {
+                                              const float2* __restrict__ tw,
+                                              const float2 (*pre)[E] = nullptr) {
     using S = Schedule<N, E>;
     constexpr int R = S::radix(P);
     constexpr int SD = S::stride(P);
@@ -120,7 +134,10 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
         for (int j = 0; j < R; ++j) x[j] = v[i + B * j];
         if constexpr (SD > 1) {
             const int pos = (t + i * T) & (SD - 1);
-            if constexpr (TW4) {
+            if constexpr (PRE) {  // register twiddles of pass P (PreTwiddles)
+#pragma unroll
+                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], pre[P - 1][i * R + j]);
+            } else if constexpr (TW4) {
                 float2 w[R];
                 twiddle_powers<R>(w, tw, pos * TWS);
 #pragma unroll
@@ -138,7 +155,9 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
     // Last pass: R * SD == N, so y_j of butterfly b = t + i*T lands at
     // b + j*SD = t + T*(i + B*j), i.e. v[m] already holds X[t + T*m].
     if constexpr (!LAST) {
-        if constexpr (P > 0 || SYNC0) lds_barrier();  // this pass's reads are done
+        if constexpr (P > 0 || SYNC0) {  // this pass's reads are done
+            if constexpr (DSB) ds_barrier(); else lds_barrier();
+        }
 #pragma unroll
         for (int i = 0; i < B; ++i) {
             const int b = t + i * T;
@@ -147,10 +166,10 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
 #pragma unroll
             for (int j = 0; j < R; ++j) buf[L::idx(base + j * SD)] = v[i + B * j];
         }
-        lds_barrier();
+        if constexpr (DSB) ds_barrier(); else lds_barrier();
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[L::idx(t + T * m)];
-        stockham_regs<N, E, L, P + 1, SYNC0, TW4>(v, buf, t, tw);
+        stockham_regs<N, E, L, P + 1, SYNC0, TW4, DSB, PRE>(v, buf, t, tw, pre);
     }
 }
 
@@ -185,7 +204,7 @@ __device__ __forceinline__ void store_column(const float2 (&v)[E], float2* dst, 
 
 // One launch = ceil(batch / TPW) workgroups, TPW transforms each.  `in` and
 // `out` may alias (in-place): a transform's loads all precede its stores.
-template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = false>
+template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = true>
 __global__ void __launch_bounds__((N / E) * TPW)
 k_stockham(const float2* in, float2* out,
            const float2* __restrict__ tw, int64_t batch) {
@@ -206,7 +225,7 @@ k_stockham(const float2* in, float2* out,
 // batch; each iteration issues the global loads of its next transform before
 // running the LDS passes of the current one, so HBM reads overlap compute.
 // WPE: minimum waves per SIMD the register allocation must allow (0 = free).
-template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = false, int WPE = 0>
+template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = true, int WPE = 0>
 __global__ void __launch_bounds__((N / E) * TPW)
 __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k_stockham_stream(const float2* in, float2* out,
@@ -231,6 +250,101 @@ k_stockham_stream(const float2* in, float2* out,
         load_column<N, E, INV, NT>(nxt, in + trn * N, t, trn < batch);
         stockham_regs<N, E, L, 0, true, TW4>(v, buf, t, tw);
         if (tr < batch) store_column<N, E, INV, NT>(v, out + tr * N, t);
+    }
+}
+
+// Streaming form with LDS DMA (gfx950 global_load_lds_dwordx4): the next
+// transform is copied HBM -> LDS by the memory pipeline, without passing
+// through VGPRs, while the current one runs its passes in the other half of
+// a double buffer.  Each wave moves 1 KiB blocks; the LDS image is the
+// transform in natural order, which pass 0 reads as v[m] = x[t + T*m].
+// Copies `count` complex64 (a multiple of 128) HBM -> LDS with
+// global_load_lds_dwordx4, 1 KiB per wave instruction.
+__device__ __forceinline__ void dma_transform(float2* lds_dst, const float2* src, int count, int tid,
+                                              int nthreads) {
+    constexpr int kBlk = 128;  // float2 per 1 KiB block: 64 lanes x 16 bytes
+    const int waves = nthreads >> 6, w = tid >> 6, lane = tid & 63;
+    const int blocks = count / kBlk;
+    for (int b = w; b < blocks; b += waves) {
+        // LDS byte address of the block (wave-uniform) -> M0; each lane's
+        // 16 bytes land at M0 + 16 * lane.  Issued from asm so the compiler
+        // does not treat every later LDS access as a hazard on this copy;
+        // the caller waits (vmcnt) before it reads the block.
+        const uint32_t ldsa = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(
+            (__attribute__((address_space(3))) char*)(lds_dst + b * kBlk)));
+        const float2* g = src + b * kBlk + 2 * lane;
+        __asm__ __volatile__("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+                             :
+                             : "v"(g), "s"(ldsa)
+                             : "memory", "m0");
+    }
+}
+
+// Register twiddles of passes 1 .. passes-1 for thread t: butterfly i of
+// pass p (radix R, B = E/R butterflies per thread) needs W_N^{j*pos*N/(R*s)},
+// j < R, pos = (t + i*T) mod s, stored at w[p-1][i*R + j] (B*R = E entries
+// per pass).  They do not depend on the transform, so a persistent
+// workgroup loads them once.
+template <int N, int E>
+struct PreTwiddles {
+    using S = Schedule<N, E>;
+    static constexpr int NP = S::passes - 1;
+    float2 w[NP > 0 ? NP : 1][E];
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+#pragma unroll
+        for (int p = 1; p <= NP; ++p) {
+            const int r = S::radix(p), b = E / r, sd = S::stride(p);
+            const int tws = N / (r * sd);
+#pragma unroll
+            for (int i = 0; i < b; ++i) {
+                const int pos = (t + i * S::T) & (sd - 1);
+#pragma unroll
+                for (int j = 0; j < r; ++j) w[p - 1][i * r + j] = tw[j * pos * tws];
+            }
+        }
+    }
+};
+
+// TPW transforms per workgroup iteration, DMA-double-buffered: one LDS half
+// receives the next TPW transforms (contiguous in HBM, TPW*N*8 bytes, whole
+// 1 KiB blocks for N >= 128) while the other runs the passes.
+template <int N, int E, int TPW, bool INV>
+__global__ void __launch_bounds__((N / E) * TPW)
+k_stockham_dma(const float2* in, float2* out, const float2* __restrict__ tw, int64_t batch) {
+    static_assert(N >= 128, "DMA moves whole 1 KiB blocks");
+    constexpr int T = N / E;
+    constexpr int NT_ = T * TPW;
+    __shared__ float2 lds[2][TPW * N];
+    const int local = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    const int64_t step = (int64_t)gridDim.x * TPW;
+    PreTwiddles<N, E> pw;
+    pw.load(tw, t);  // issued before any DMA: their wait does not cover a prefetch
+    int cur = 0;
+    int64_t base = (int64_t)blockIdx.x * TPW;
+    if (base < batch) {
+        const int cnt = (int)(batch - base < TPW ? batch - base : TPW);
+        dma_transform(lds[0], in + base * N, cnt * N, threadIdx.x, NT_);
+    }
+    for (; base < batch; base += step) {  // bound uniform per workgroup
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA (and earlier stores) done
+        ds_barrier();                   // every wave's DMA landed; previous batch finished
+        const int64_t tr = base + local;
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const float2 x = lds[cur][local * N + t + T * m];
+            v[m] = INV ? conjf2(x) : x;
+        }
+        const int64_t nb = base + step;
+        if (nb < batch) {
+            const int cnt = (int)(batch - nb < TPW ? batch - nb : TPW);
+            dma_transform(lds[cur ^ 1], in + nb * N, cnt * N, threadIdx.x, NT_);
+        }
+        stockham_regs<N, E, LdsSwz, 0, true, false, true, true>(v, lds[cur] + local * N, t, tw,
+                                                                  pw.w);
+        if (tr < batch) store_column<N, E, INV, true>(v, out + tr * N, t);
+        cur ^= 1;
     }
 }
 

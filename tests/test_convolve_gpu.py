@@ -23,8 +23,13 @@ def test_convolver_matches_oracle_and_fp64(gpu, oracle, n, k, s):
     assert cv.blockSize == n and cv.hopSize == n - k + 1 and cv.kernelLen == k
     y = cv.apply(sig)
     assert y.size == s + k - 1
-    assert oracle.rel_l2(y, oracle.f64_convolve(sig, taps)) < 1e-6
-    assert oracle.rel_l2(y, oracle.cpu_convolve(sig, taps, n)) < 1e-6
+    ref64 = oracle.f64_convolve(sig, taps)
+    cpu = oracle.cpu_convolve(sig, taps, n)
+    assert oracle.rel_l2(y, ref64) < 1e-6
+    # 1e-6 to the CPU path; where that path is itself further than 1e-6 from
+    # FP64 (hop = 2 at N = 16384: 20,000 overlapping blocks) its own error is
+    # the slack
+    assert oracle.rel_l2(y, cpu) < 1e-6 + oracle.rel_l2(cpu, ref64)
 
 
 def test_delta_filter_is_identity(gpu, oracle):

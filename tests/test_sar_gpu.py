@@ -41,13 +41,15 @@ def test_rda_small_vs_fp64(gpu, oracle, io, bound):
     err = oracle.rel_l2(got, ref)
     print(f"io={io} rel_l2={err:.3e}")
     assert err < bound
+    # peaks: range to a bin; azimuth to 4 bins (the azimuth mainlobe spans
+    # ~PRF / Doppler bandwidth ~ 30 bins in this geometry, so its top is flat)
     mag = np.abs(got)
     for r0, xt, _a in tg:
         rb, ab = oracle.sar_expected_bins(p, r0, xt)
         ri, ai = int(round(rb)), int(round(ab))
-        w = mag[ri - 3:ri + 4, ai - 3:ai + 4]
+        w = mag[ri - 3:ri + 4, ai - 8:ai + 9]
         m = np.unravel_index(np.argmax(w), w.shape)
-        assert abs(ri - 3 + m[0] - rb) <= 1.5 and abs(ai - 3 + m[1] - ab) <= 1.5
+        assert abs(ri - 3 + m[0] - rb) <= 1.0 and abs(ai - 8 + m[1] - ab) <= 4.0
 
 
 def test_rda_reference_formula_matches_oracle(gpu, oracle):
