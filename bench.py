@@ -37,6 +37,10 @@ def parse():
     p.add_argument("--kernel", default=None, help="pin a kernel variant by name")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the MFMA / ragged / convolve / SAR measurements")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="only run 4 launches of a plain 128 MiB copy (elementwise, known "
+                        "bytes) and 4 of the FFT step, for rocprofv3 --pmc FETCH_SIZE / "
+                        "WRITE_SIZE runs")
     return p.parse_args()
 
 
@@ -262,11 +266,37 @@ def main():
                                                                       device="cuda")))
     torch.cuda.synchronize()
 
+    # launch through a pre-bound ctypes call: per-step Python work is one
+    # foreign call, so back-to-back launches keep the GPU fed
+    import ctypes
+    from metalfft_amd import _lib
+    L = _lib.load()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    calls = [(fft._plan, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(d.data_ptr()),
+              ctypes.c_int64(batch), ctypes.c_int(m.FORWARD), stream) for s, d in sets]
+    execute = L.mifft_execute_device
+
     def step(i):
-        src, dst = sets[i % len(sets)]
-        fft.execute(src, dst, batch=batch)
+        execute(*calls[i % len(calls)])
+
+    if a.traffic_probe:
+        # copy reference: complex64 out = in * 1 over the same 128 MiB sets,
+        # 128 MiB read + 128 MiB written per launch
+        for i in range(4):
+            src, dst = sets[i % len(sets)]
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        for i in range(4):
+            step(i)
+        torch.cuda.synchronize()
+        print(json.dumps({"traffic_probe": True, "kernel": fft.kernel_name,
+                          "copy_bytes_read": 8 * n * batch, "copy_bytes_written": 8 * n * batch}))
+        return
 
     ms, wall = time_gpu(step, a.steps, a.warmup, torch)
+    rc = execute(*calls[0])
+    if rc != 0:
+        _lib.check(rc)
     value = batch / (ms * 1e-3)
     alg_bytes = 16 * n * batch
     achieved = alg_bytes / (ms * 1e-3) / 1e9

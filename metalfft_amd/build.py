@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmifft.so")
+SELFTEST = os.path.join(HERE, "mifft_host_selftest")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 ARCH = "gfx950"
@@ -51,8 +52,17 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ mirror self-test, linked against the library (rpath: its directory)
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "host", "selftest.cpp"), "-o", SELFTEST + ".tmp",
+           "-L", HERE, "-lmifft", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"self-test build failed:\n{r.stderr}")
+    os.replace(SELFTEST + ".tmp", SELFTEST)
     if verbose:
-        print(f"built {LIB} from {len(objs)} sources", file=sys.stderr)
+        print(f"built {LIB} from {len(objs)} sources (+ {os.path.basename(SELFTEST)})",
+              file=sys.stderr)
     return LIB
 
 

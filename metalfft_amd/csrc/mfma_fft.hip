@@ -113,8 +113,9 @@ __device__ __forceinline__ f4 gemm_half(typename MT<BF>::v8 a_hi, typename MT<BF
     return MT<BF>::mma(a_hi, b_hi, acc);
 }
 
+// Register budget for three resident workgroups per CU (3 waves per SIMD).
 template <bool BF, bool SPLIT, bool INV, class IO>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __restrict__ tw,
                int64_t batch, float out_scale) {
     constexpr int N = 4096;
@@ -145,18 +146,16 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         split8<BF, SPLIT>(fr, are_hi, are_lo);
         split8<BF, SPLIT>(fi, aim_hi, aim_lo);
     }
-    // stage twiddles of passes 1 (s = 16) and 2 (s = 256): W_4096^{j*pos*N/(16 s)}
-    float2 twr[2][4][4];
+    // stage twiddles W_4096^{j*pos*N/(16 s)}: pass 1 (s = 16) has pos = c for
+    // every tile, pass 2 (s = 256) pos = 16*(4w + tile) + c
+    float2 tw1[4], tw2[4][4];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int s = q == 0 ? 16 : 256;
-        const int tws = N / (16 * s);
+    for (int i = 0; i < 4; ++i) tw1[i] = tw[(4 * g + i) * c * 16];
 #pragma unroll
-        for (int t4 = 0; t4 < 4; ++t4) {
-            const int pos = (16 * (4 * w + t4) + c) & (s - 1);
+    for (int t4 = 0; t4 < 4; ++t4) {
+        const int pos = 16 * (4 * w + t4) + c;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) twr[q][t4][i] = tw[(4 * g + i) * pos * tws];
-        }
+        for (int i = 0; i < 4; ++i) tw2[t4][i] = tw[(4 * g + i) * pos];
     }
 
     constexpr int kElemBytes = sizeof(typename IO::T);
@@ -208,7 +207,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int j = 4 * g + i;
-                        v[t4][i] = cmul(lds[midx(b + 256 * j)], twr[p - 1][t4][i]);
+                        v[t4][i] = cmul(lds[midx(b + 256 * j)], p == 1 ? tw1[i] : tw2[t4][i]);
                     }
                 }
             }
