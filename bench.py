@@ -6,7 +6,7 @@ rotate over >= 1 GiB so the 256 MiB Infinity Cache cannot hold a step's
 working set.  Prints one JSON line (driver contract); extra keys carry the
 roofline, the CPU baseline and the reduced-precision forms.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 10]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20]
 """
 from __future__ import annotations
 
@@ -28,12 +28,15 @@ METRIC = "complex FFTs/s + % HBM roofline, N=4096 b=4096; FP16-MFMA vs FP32 at s
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
-    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=4096)
     p.add_argument("--batch", type=int, default=4096)
     p.add_argument("--sets", type=int, default=4, help="rotating in/out buffer sets")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--launch", choices=("stream", "graph"), default="stream",
+                   help="timed region: K launches from a Python loop (default) or one hipGraph "
+                        "replay of the K launches; the other is reported beside it")
     p.add_argument("--kernel", default=None, help="pin a kernel variant by name")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the MFMA / ragged / convolve / SAR measurements")
@@ -45,8 +48,10 @@ def parse():
 
 
 def time_gpu(fn, steps, warmup, torch):
+    """Stream loop: `steps` back-to-back launches from Python, HIP events on
+    the launch stream around them."""
     for i in range(warmup):
-        fn(i)
+        fn(i, None)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -54,11 +59,39 @@ def time_gpu(fn, steps, warmup, torch):
     t0 = time.perf_counter()
     e0.record(stream)
     for i in range(steps):
-        fn(i)
+        fn(i, None)
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return e0.elapsed_time(e1) / steps, wall
+
+
+def time_graph(fn, steps, warmup, torch):
+    """The same `steps` launches captured once into a hipGraph and replayed:
+    the GPU runs them back to back with no host launch jitter between them.
+    Warm-up launches run eagerly first (plans and occupancy queries settle
+    outside the capture), then one untimed replay; HIP events on the replay
+    stream bracket the timed replay."""
+    for i in range(warmup):
+        fn(i, None)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        capture = torch.cuda.current_stream().cuda_stream
+        for i in range(steps):
+            fn(i, capture)
+    graph.replay()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    graph.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall, graph
 
 
 def cpu_baseline(n, batch, torch_seed_data):
@@ -276,8 +309,13 @@ def main():
               ctypes.c_int64(batch), ctypes.c_int(m.FORWARD), stream) for s, d in sets]
     execute = L.mifft_execute_device
 
-    def step(i):
-        execute(*calls[i % len(calls)])
+    def step(i, on_stream):
+        c = calls[i % len(calls)]
+        if on_stream is not None:
+            c = c[:5] + (ctypes.c_void_p(on_stream),)
+        rc = execute(*c)
+        if rc != 0:
+            _lib.check(rc)
 
     if a.traffic_probe:
         # copy reference: complex64 out = in * 1 over the same 128 MiB sets,
@@ -287,16 +325,16 @@ def main():
             torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         for i in range(4):
-            step(i)
+            step(i, None)
         torch.cuda.synchronize()
         print(json.dumps({"traffic_probe": True, "kernel": fft.kernel_name,
                           "copy_bytes_read": 8 * n * batch, "copy_bytes_written": 8 * n * batch}))
         return
 
-    ms, wall = time_gpu(step, a.steps, a.warmup, torch)
-    rc = execute(*calls[0])
-    if rc != 0:
-        _lib.check(rc)
+    ms_loop, wall_loop = time_gpu(step, a.steps, a.warmup, torch)
+    ms_graph, wall_graph, graph = time_graph(step, a.steps, a.warmup, torch)
+    del graph
+    ms, wall = (ms_graph, wall_graph) if a.launch == "graph" else (ms_loop, wall_loop)
     value = batch / (ms * 1e-3)
     alg_bytes = 16 * n * batch
     achieved = alg_bytes / (ms * 1e-3) / 1e9
@@ -352,6 +390,9 @@ def main():
         "gflops_5nlog2n": 5 * n * np.log2(n) * value / 1e9,
         "device": m.device_name(),
         "wall_s_timed": wall,
+        "launch": {"timed": "hipGraph replay of the K launches" if a.launch == "graph"
+                   else "K launches from a Python loop",
+                   "stream_loop_ms_per_step": ms_loop, "graph_replay_ms_per_step": ms_graph},
         "extras": extras,
     }
     print(json.dumps(line))

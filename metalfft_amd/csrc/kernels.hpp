@@ -20,6 +20,10 @@ struct StockhamVariant {
 
 const StockhamVariant* stockham_variants(int* count);
 const StockhamVariant* find_stockham(int n, int e);
+// Plan twiddle memory: W_N^k for k < n, then the packed per-pass bases rows
+// of the radix-16 schedule (PassTable<n, 16, 0>) the "_pt" variants read.
+// twiddle_table_len(n) entries; launch_twiddles fills all of them.
+int twiddle_table_len(int n);
 hipError_t launch_twiddles(float2* tw, int n, hipStream_t stream);
 
 }  // namespace mifft

@@ -266,7 +266,8 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
         delete p;
         return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
     }
-    if ((s = check_hip(hipMalloc(&p->twiddles, sizeof(float2) * n), "hipMalloc")) != MIFFT_OK) {
+    if ((s = check_hip(hipMalloc(&p->twiddles, sizeof(float2) * twiddle_table_len((int)n)),
+                       "hipMalloc")) != MIFFT_OK) {
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
     }

@@ -42,7 +42,7 @@ struct mifft_plan {
     int precision = 0;  // mifft_precision
     int io = 0;         // mifft_io
     const mifft::StockhamVariant* variant = nullptr;
-    float2* twiddles = nullptr;  // W_N^k, k < N, FP64-generated, FP32-stored
+    float2* twiddles = nullptr;  // W_N^k, k < N, then pass rows (twiddle_table_len)
     hipStream_t stream = nullptr;
     mifft::DeviceBuffer in, out;
 };

@@ -11,7 +11,11 @@
 //                   registers during the LDS passes;
 //   _dma          : persistent grid, next transforms copied HBM -> LDS by
 //                   global_load_lds DMA into the other half of a double
-//                   buffer, twiddles held in registers (N >= 128).
+//                   buffer, twiddles held in registers (N >= 128);
+//   _swz_nt_pt    : _swz_nt whose threads load their twiddle table entries
+//                   for every pass right after the transform's data, as
+//                   16-byte reads of packed per-pass rows (PassTable), so no
+//                   pass waits on an L2 round trip for them.
 // All read twiddles as W^k, W^2k, W^4k, W^8k and derive the other powers.
 // The autotuner picks one per size (autotune.hip).
 #include "kernels.hpp"
@@ -59,7 +63,7 @@ static hipError_t launch_dma(const float2* in, float2* out, const float2* tw, in
     return hipGetLastError();
 }
 
-template <int N, int E, class L, bool NT, bool PERS>
+template <int N, int E, class L, bool NT, bool PERS, int TWM = 0>
 static hipError_t launch_stockham(const float2* in, float2* out, const float2* tw, int64_t batch,
                                   bool inverse, hipStream_t stream) {
     constexpr int TPW = tpw_for<N, E>();
@@ -89,11 +93,11 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
     } else {
         if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
         if (inverse)
-            hipLaunchKernelGGL((k_stockham<N, E, TPW, true, L, NT>), dim3((unsigned)blocks),
+            hipLaunchKernelGGL((k_stockham<N, E, TPW, true, L, NT, true, TWM>), dim3((unsigned)blocks),
                                dim3(threads), 0, stream, in, out, tw, batch);
         else
-            hipLaunchKernelGGL((k_stockham<N, E, TPW, false, L, NT>), dim3((unsigned)blocks),
-                               dim3(threads), 0, stream, in, out, tw, batch);
+            hipLaunchKernelGGL((k_stockham<N, E, TPW, false, L, NT, true, TWM>),
+                               dim3((unsigned)blocks), dim3(threads), 0, stream, in, out, tw, batch);
     }
     return hipGetLastError();
 }
@@ -106,6 +110,11 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
 #define MIFFT_VP(N, E)                                                                          \
     {N, E, tpw_for<N, E>(), &launch_stockham<N, E, LdsSwz, true, true>,                         \
      "stockham_r" #E "_" #N "_stream"}
+// packed bases rows (PassTable<N, 16, 0>, after the N-entry table) loaded
+// with the transform's data
+#define MIFFT_VT(N)                                                                             \
+    {N, 16, tpw_for<N, 16>(), &launch_stockham<N, 16, LdsSwz, true, false, 2>,                  \
+     "stockham_r16_" #N "_swz_nt_pt"}
 #define MIFFT_VD(N, E) \
     {N, E, tpw_for<N, E>(), &launch_dma<N, E>, "stockham_r" #E "_" #N "_dma"}
 
@@ -127,11 +136,14 @@ static const StockhamVariant kVariants[] = {
     MIFFT_V(8192, 8),  MIFFT_V(8192, 16), MIFFT_VS(8192, 16), MIFFT_VP(8192, 8),  MIFFT_VP(8192, 16),
     MIFFT_VD(8192, 8), MIFFT_VD(8192, 16),
     MIFFT_V(16384, 16), MIFFT_VS(16384, 16), MIFFT_VP(16384, 16),
+    MIFFT_VT(64),      MIFFT_VT(128),     MIFFT_VT(256),      MIFFT_VT(512),      MIFFT_VT(1024),
+    MIFFT_VT(2048),    MIFFT_VT(4096),    MIFFT_VT(8192),     MIFFT_VT(16384),
 };
 #undef MIFFT_V
 #undef MIFFT_VS
 #undef MIFFT_VP
 #undef MIFFT_VD
+#undef MIFFT_VT
 
 const StockhamVariant* stockham_variants(int* count) {
     *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
@@ -157,10 +169,52 @@ __global__ void k_twiddles(float2* __restrict__ tw, int n) {
     }
 }
 
+namespace {
+template <int N>
+constexpr int pass_rows() { return PassTable<N, 16, 0>::size; }
+
+template <int N>
+hipError_t launch_pass_rows(float2* tab, hipStream_t stream) {
+    constexpr int size = pass_rows<N>();
+    if constexpr (size > 0)
+        hipLaunchKernelGGL((k_pass_table<N, 16, 0>), dim3((size + 255) / 256), dim3(256), 0, stream,
+                           tab);
+    return hipGetLastError();
+}
+}  // namespace
+
+int twiddle_table_len(int n) {
+    switch (n) {
+        case 64: return 64 + pass_rows<64>();
+        case 128: return 128 + pass_rows<128>();
+        case 256: return 256 + pass_rows<256>();
+        case 512: return 512 + pass_rows<512>();
+        case 1024: return 1024 + pass_rows<1024>();
+        case 2048: return 2048 + pass_rows<2048>();
+        case 4096: return 4096 + pass_rows<4096>();
+        case 8192: return 8192 + pass_rows<8192>();
+        case 16384: return 16384 + pass_rows<16384>();
+        default: return n;
+    }
+}
+
 hipError_t launch_twiddles(float2* tw, int n, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_twiddles, dim3((n + 255) / 256), dim3(256), 0, stream, tw, n);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (n) {
+        case 64: return launch_pass_rows<64>(tw + n, stream);
+        case 128: return launch_pass_rows<128>(tw + n, stream);
+        case 256: return launch_pass_rows<256>(tw + n, stream);
+        case 512: return launch_pass_rows<512>(tw + n, stream);
+        case 1024: return launch_pass_rows<1024>(tw + n, stream);
+        case 2048: return launch_pass_rows<2048>(tw + n, stream);
+        case 4096: return launch_pass_rows<4096>(tw + n, stream);
+        case 8192: return launch_pass_rows<8192>(tw + n, stream);
+        case 16384: return launch_pass_rows<16384>(tw + n, stream);
+        default: return hipSuccess;
+    }
 }
 
 }  // namespace mifft

@@ -81,29 +81,92 @@ __device__ __forceinline__ void gstore(float2* p, float2 v) {
 }
 
 // ------------------------------------------------------------ twiddles
-// w[j] = W^{j*k} for j < R from the table entries W^k, W^{2k}, W^{4k}, W^{8k}
-// (R/4.. loads instead of R - 1); every other power is a product of at most
-// three table entries, so it carries at most three FP32 roundings beyond the
-// table's own (~1e-7 relative).
+// Number of table entries a radix-R butterfly reads: W^k, W^2k, W^4k, W^8k
+// for R = 16 (log2 R of them).
+constexpr int twiddle_bases(int r) { return r >= 16 ? 4 : r >= 8 ? 3 : r >= 4 ? 2 : r >= 2 ? 1 : 0; }
+
+// w[j] = W^{j*k} for j < R from b[q] = W^{(2^q)*k}; every other power is a
+// product of at most three of them, so it carries at most three FP32
+// roundings beyond the table's own (~1e-7 relative).
 template <int R>
-__device__ __forceinline__ void twiddle_powers(float2 (&w)[R], const float2* __restrict__ tw, int k) {
+__device__ __forceinline__ void powers_from_bases(float2 (&w)[R], const float2* b) {
     w[0] = make_float2(1.f, 0.f);
-    if constexpr (R >= 2) w[1] = tw[k];
+    if constexpr (R >= 2) w[1] = b[0];
     if constexpr (R >= 4) {
-        w[2] = tw[2 * k];
+        w[2] = b[1];
         w[3] = cmul(w[1], w[2]);
     }
     if constexpr (R >= 8) {
-        w[4] = tw[4 * k];
+        w[4] = b[2];
         w[5] = cmul(w[1], w[4]);
         w[6] = cmul(w[2], w[4]);
         w[7] = cmul(w[3], w[4]);
     }
     if constexpr (R >= 16) {
-        w[8] = tw[8 * k];
+        w[8] = b[3];
 #pragma unroll
         for (int j = 9; j < 16; ++j) w[j] = cmul(w[j - 8], w[8]);
     }
+}
+
+// w[j] = W^{j*k} for j < R from the table entries W^k, W^2k, W^4k, W^8k
+// (log2 R loads instead of R - 1).
+template <int R>
+__device__ __forceinline__ void twiddle_powers(float2 (&w)[R], const float2* __restrict__ tw, int k) {
+    float2 b[4];
+#pragma unroll
+    for (int q = 0; q < twiddle_bases(R); ++q) b[q] = tw[k << q];
+    powers_from_bases<R>(w, b);
+}
+
+// Row-packed per-pass tables, stored right after the N-entry table W_N^k:
+// for pass p >= 1 (radix R, stride s, k = pos*N/(R*s)) row pos < s holds
+//   ROW == 0 (bases):  W^k, W^{2k}, W^{4k}, W^{8k}      (4 entries)
+//   ROW == 1 (powers): W^{jk}, j = 0 .. R-1              (R entries)
+// each entry the FP64 value of W_N^{m mod N} rounded once, so a thread gets
+// its butterfly's twiddles with contiguous 16-byte loads (one or two rows of
+// 32/128 bytes) instead of log2 R scattered table reads.
+template <int N, int E, int ROW>
+struct PassTable {
+    using S = Schedule<N, E>;
+    static constexpr int width(int p) { return ROW == 0 ? 4 : S::radix(p); }
+    static constexpr int offset(int p) {
+        return p <= 1 ? 0 : offset(p - 1) + S::stride(p - 1) * width(p - 1);
+    }
+    static constexpr int size = offset(S::passes);
+};
+
+// Reads n2 float4 = 2*n2 complex entries starting at row (16-byte aligned).
+template <int n2>
+__device__ __forceinline__ void load_row(float2* w, const float2* row) {
+    const float4* r4 = reinterpret_cast<const float4*>(row);
+#pragma unroll
+    for (int q = 0; q < n2; ++q) {
+        const float4 a = r4[q];
+        w[2 * q] = make_float2(a.x, a.y);
+        w[2 * q + 1] = make_float2(a.z, a.w);
+    }
+}
+
+// Fills PassTable<N, E, ROW> at tab (= tw + N): column c of row pos of pass
+// p is W_N^{m mod N}, m = k << c (bases) or c*k (powers), evaluated in FP64
+// with sincospi and rounded once, as the N-entry table is (bases rows equal
+// tw[k << c] bit for bit).
+template <int N, int E, int ROW>
+__global__ void k_pass_table(float2* __restrict__ tab) {
+    using S = Schedule<N, E>;
+    using PTab = PassTable<N, E, ROW>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= PTab::size) return;
+    int p = 1;
+    while (p + 1 < S::passes && i >= PTab::offset(p + 1)) ++p;
+    const int w = PTab::width(p), rel = i - PTab::offset(p);
+    const int pos = rel / w, c = rel % w;
+    const int64_t k = (int64_t)pos * (N / (S::radix(p) * S::stride(p)));
+    const int64_t m = (ROW == 0 ? (k << c) : (int64_t)c * k) % N;
+    double sn, cs;
+    sincospi(-2.0 * (double)m / (double)N, &sn, &cs);
+    tab[i] = make_float2((float)cs, (float)sn);
 }
 
 // ------------------------------------------------------------ the passes
@@ -112,10 +175,14 @@ __device__ __forceinline__ void twiddle_powers(float2 (&w)[R], const float2* __r
 // buffer was read by a previous transform in the same workgroup).
 // TW4: derive twiddle powers from four table loads (twiddle_powers) instead
 // of loading all R - 1.
-// PRE: twiddles of passes >= 1 come from `pre` (registers, PreTwiddles).
+// PRE: twiddles of passes >= 1 come from `pre` (registers): 1 = all R - 1
+// powers per butterfly (PreTwiddles), 2 = the log2 R table entries per
+// butterfly, powers derived at use (PreBases); or from the row-packed pass
+// tables after the N-entry table: 3 = bases rows (powers derived),
+// 4 = powers rows (PassTable).
 // DSB: use ds_barrier (LDS-only ordering) instead of the fence barrier.
 template <int N, int E, class L, int P, bool SYNC0, bool TW4 = true, bool DSB = false,
-          bool PRE = false>
+          int PRE = 0>
 __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t,
                                               const float2* __restrict__ tw,
                                               const float2 (*pre)[E] = nullptr) {
@@ -134,9 +201,28 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
         for (int j = 0; j < R; ++j) x[j] = v[i + B * j];
         if constexpr (SD > 1) {
             const int pos = (t + i * T) & (SD - 1);
-            if constexpr (PRE) {  // register twiddles of pass P (PreTwiddles)
+            if constexpr (PRE == 1) {  // register twiddles of pass P (PreTwiddles)
 #pragma unroll
                 for (int j = 1; j < R; ++j) x[j] = cmul(x[j], pre[P - 1][i * R + j]);
+            } else if constexpr (PRE == 2) {  // register table entries (PreBases)
+                float2 w[R];
+                powers_from_bases<R>(w, &pre[P - 1][i * twiddle_bases(R)]);
+#pragma unroll
+                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
+            } else if constexpr (PRE == 3) {  // packed bases row, powers derived
+                constexpr int off = N + PassTable<N, E, 0>::offset(P);
+                float2 b[4];
+                load_row<(twiddle_bases(R) + 1) / 2>(b, tw + off + pos * 4);
+                float2 w[R];
+                powers_from_bases<R>(w, b);
+#pragma unroll
+                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
+            } else if constexpr (PRE == 4) {  // packed powers row
+                constexpr int off = N + PassTable<N, E, 1>::offset(P);
+                float2 w[R];
+                load_row<R / 2>(w, tw + off + pos * R);
+#pragma unroll
+                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
             } else if constexpr (TW4) {
                 float2 w[R];
                 twiddle_powers<R>(w, tw, pos * TWS);
@@ -202,9 +288,83 @@ __device__ __forceinline__ void store_column(const float2 (&v)[E], float2* dst, 
     }
 }
 
+// Register twiddles of passes 1 .. passes-1 for thread t: butterfly i of
+// pass p (radix R, B = E/R butterflies per thread) needs W_N^{j*pos*N/(R*s)},
+// j < R, pos = (t + i*T) mod s, stored at w[p-1][i*R + j] (B*R = E entries
+// per pass).  They do not depend on the transform, so a persistent
+// workgroup loads them once.
+template <int N, int E>
+struct PreTwiddles {
+    using S = Schedule<N, E>;
+    static constexpr int NP = S::passes - 1;
+    float2 w[NP > 0 ? NP : 1][E];
+    template <int p = 1>
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+        if constexpr (p <= NP) {
+            constexpr int r = S::radix(p), b = E / r, sd = S::stride(p);
+            constexpr int tws = N / (r * sd);
+#pragma unroll
+            for (int i = 0; i < b; ++i) {
+                const int pos = (t + i * S::T) & (sd - 1);
+#pragma unroll
+                for (int j = 0; j < r; ++j) w[p - 1][i * r + j] = tw[j * pos * tws];
+            }
+            load<p + 1>(tw, t);
+        }
+    }
+};
+
+// Register table entries of passes 1 .. passes-1 for thread t: butterfly i
+// of pass p needs W^{(2^q) k}, q < log2 R, k = pos*N/(R*s), stored at
+// w[p-1][i*log2(R) + q] (at most E entries per pass); powers_from_bases
+// expands them at use.  Loaded with the transform's data, their L2 latency
+// leaves the pass-to-pass critical path.
+template <int N, int E>
+struct PreBases {
+    using S = Schedule<N, E>;
+    static constexpr int NP = S::passes - 1;
+    float2 w[NP > 0 ? NP : 1][E];
+    template <int p = 1>
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+        if constexpr (p <= NP) {
+            constexpr int r = S::radix(p), b = E / r, sd = S::stride(p), nb = twiddle_bases(r);
+            constexpr int tws = N / (r * sd);
+#pragma unroll
+            for (int i = 0; i < b; ++i) {
+                const int k = ((t + i * S::T) & (sd - 1)) * tws;
+#pragma unroll
+                for (int q = 0; q < nb; ++q) w[p - 1][i * nb + q] = tw[k << q];
+            }
+            load<p + 1>(tw, t);
+        }
+    }
+    // the same entries from the bases rows of PassTable<N, E, 0>
+    template <int p = 1>
+    __device__ __forceinline__ void load_packed(const float2* __restrict__ tw, int t) {
+        if constexpr (p <= NP) {
+            constexpr int r = S::radix(p), b = E / r, sd = S::stride(p), nb = twiddle_bases(r);
+            constexpr int off = N + PassTable<N, E, 0>::offset(p);
+#pragma unroll
+            for (int i = 0; i < b; ++i) {
+                const int pos = (t + i * S::T) & (sd - 1);
+                float2 row[4];
+                load_row<(nb + 1) / 2>(row, tw + off + pos * 4);
+#pragma unroll
+                for (int q = 0; q < nb; ++q) w[p - 1][i * nb + q] = row[q];
+            }
+            load_packed<p + 1>(tw, t);
+        }
+    }
+};
+
 // One launch = ceil(batch / TPW) workgroups, TPW transforms each.  `in` and
 // `out` may alias (in-place): a transform's loads all precede its stores.
-template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = true>
+// TWM, twiddles of passes >= 1: 0 = table reads at each pass (TW4 or all
+// R - 1), 1 = log2 R entries per butterfly loaded right after the
+// transform's data (PreBases), 2 = the same from the packed bases rows,
+// 3 = packed bases rows at each pass, 4 = packed powers rows at each pass.
+// Modes 2-4 need the PassTable rows after the N-entry table.
+template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = true, int TWM = 0>
 __global__ void __launch_bounds__((N / E) * TPW)
 k_stockham(const float2* in, float2* out,
            const float2* __restrict__ tw, int64_t batch) {
@@ -217,7 +377,13 @@ k_stockham(const float2* in, float2* out,
     float2* buf = lds + local * L::len(N);
     float2 v[E];
     load_column<N, E, INV, NT>(v, in + tr * N, t, active);
-    stockham_regs<N, E, L, 0, false, TW4>(v, buf, t, tw);
+    if constexpr (TWM == 1 || TWM == 2) {
+        PreBases<N, E> pb;
+        if constexpr (TWM == 1) pb.load(tw, t); else pb.load_packed(tw, t);
+        stockham_regs<N, E, L, 0, false, TW4, false, 2>(v, buf, t, tw, pb.w);
+    } else {
+        stockham_regs<N, E, L, 0, false, TW4, false, TWM == 0 ? 0 : TWM>(v, buf, t, tw);
+    }
     if (active) store_column<N, E, INV, NT>(v, out + tr * N, t);
 }
 
@@ -225,7 +391,10 @@ k_stockham(const float2* in, float2* out,
 // batch; each iteration issues the global loads of its next transform before
 // running the LDS passes of the current one, so HBM reads overlap compute.
 // WPE: minimum waves per SIMD the register allocation must allow (0 = free).
-template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = true, int WPE = 0>
+// TWM as in k_stockham; with 1 or 2 the table entries stay in registers for
+// the whole loop (they depend on the thread only).
+template <int N, int E, int TPW, bool INV, class L, bool NT, bool TW4 = true, int WPE = 0,
+          int TWM = 0>
 __global__ void __launch_bounds__((N / E) * TPW)
 __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k_stockham_stream(const float2* in, float2* out,
@@ -239,6 +408,9 @@ k_stockham_stream(const float2* in, float2* out,
     float2 nxt[E];
     int64_t tr = (int64_t)blockIdx.x * TPW + local;
     load_column<N, E, INV, NT>(nxt, in + tr * N, t, tr < batch);
+    PreBases<N, E> pb;
+    if constexpr (TWM == 1) pb.load(tw, t);
+    if constexpr (TWM == 2) pb.load_packed(tw, t);
     // the loop bound depends on blockIdx only: every thread of a workgroup
     // runs the same iterations, so barriers stay uniform
     for (int64_t base = (int64_t)blockIdx.x * TPW; base < batch; base += step) {
@@ -248,7 +420,10 @@ k_stockham_stream(const float2* in, float2* out,
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         const int64_t trn = tr + step;
         load_column<N, E, INV, NT>(nxt, in + trn * N, t, trn < batch);
-        stockham_regs<N, E, L, 0, true, TW4>(v, buf, t, tw);
+        if constexpr (TWM == 1 || TWM == 2)
+            stockham_regs<N, E, L, 0, true, TW4, false, 2>(v, buf, t, tw, pb.w);
+        else
+            stockham_regs<N, E, L, 0, true, TW4, false, TWM>(v, buf, t, tw);
         if (tr < batch) store_column<N, E, INV, NT>(v, out + tr * N, t);
     }
 }
@@ -279,31 +454,6 @@ __device__ __forceinline__ void dma_transform(float2* lds_dst, const float2* src
                              : "memory", "m0");
     }
 }
-
-// Register twiddles of passes 1 .. passes-1 for thread t: butterfly i of
-// pass p (radix R, B = E/R butterflies per thread) needs W_N^{j*pos*N/(R*s)},
-// j < R, pos = (t + i*T) mod s, stored at w[p-1][i*R + j] (B*R = E entries
-// per pass).  They do not depend on the transform, so a persistent
-// workgroup loads them once.
-template <int N, int E>
-struct PreTwiddles {
-    using S = Schedule<N, E>;
-    static constexpr int NP = S::passes - 1;
-    float2 w[NP > 0 ? NP : 1][E];
-    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
-#pragma unroll
-        for (int p = 1; p <= NP; ++p) {
-            const int r = S::radix(p), b = E / r, sd = S::stride(p);
-            const int tws = N / (r * sd);
-#pragma unroll
-            for (int i = 0; i < b; ++i) {
-                const int pos = (t + i * S::T) & (sd - 1);
-#pragma unroll
-                for (int j = 0; j < r; ++j) w[p - 1][i * r + j] = tw[j * pos * tws];
-            }
-        }
-    }
-};
 
 // TPW transforms per workgroup iteration, DMA-double-buffered: one LDS half
 // receives the next TPW transforms (contiguous in HBM, TPW*N*8 bytes, whole
@@ -341,7 +491,7 @@ k_stockham_dma(const float2* in, float2* out, const float2* __restrict__ tw, int
             const int cnt = (int)(batch - nb < TPW ? batch - nb : TPW);
             dma_transform(lds[cur ^ 1], in + nb * N, cnt * N, threadIdx.x, NT_);
         }
-        stockham_regs<N, E, LdsSwz, 0, true, false, true, true>(v, lds[cur] + local * N, t, tw,
+        stockham_regs<N, E, LdsSwz, 0, true, false, true, 1>(v, lds[cur] + local * N, t, tw,
                                                                   pw.w);
         if (tr < batch) store_column<N, E, INV, true>(v, out + tr * N, t);
         cur ^= 1;
