@@ -251,10 +251,16 @@ def extra_convolve(m, torch, count=512, S=65536, K=1023, N=4096):
     bwd = time_fn(lambda: cv.backward_device(x, gy, gx, gh, S, count), 10, 2, torch)
     fbytes = 4 * count * (S + S + K - 1)
     bbytes = 4 * count * (S + (S + K - 1) + S)
+    fp16 = {}
+    for name in ("float16Pure", "float16Storage", "float16Mixed"):
+        c16 = m.FusedConvolver(h.cpu().numpy(), N, getattr(m.Precision, name))
+        ms = time_fn(lambda: c16.apply_device(x, y, S, count), 10, 2, torch)
+        fp16[name] = {"forward_ms": ms, "forward_gbs_alg": fbytes / (ms * 1e-3) / 1e9}
+        c16.close()
     return {"signals": count, "samples": S, "taps": K, "fft": N,
             "forward_ms": fwd, "forward_gbs_alg": fbytes / (fwd * 1e-3) / 1e9,
             "backward_ms": bwd, "backward_gbs_alg": bbytes / (bwd * 1e-3) / 1e9,
-            "signals_per_s_fwd_bwd": count / ((fwd + bwd) * 1e-3)}
+            "signals_per_s_fwd_bwd": count / ((fwd + bwd) * 1e-3), "fp16_forward": fp16}
 
 
 def extra_sar(m, torch, n=4096):

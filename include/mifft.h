@@ -156,9 +156,9 @@ typedef struct mifft_convolver mifft_convolver;
 /* FusedConvolver.Precision (Sources/MetalFFT/DSP/FusedConvolver.swift:27-36). */
 typedef enum mifft_conv_precision {
     MIFFT_CONV_FP32 = 0,         /* .float32 */
-    MIFFT_CONV_FP16_PURE = 1,    /* .float16Pure    (not built: MIFFT_ERR_KERNEL_NOT_FOUND) */
-    MIFFT_CONV_FP16_STORAGE = 2, /* .float16Storage (not built: MIFFT_ERR_KERNEL_NOT_FOUND) */
-    MIFFT_CONV_FP16_MIXED = 3    /* .float16Mixed   (not built: MIFFT_ERR_KERNEL_NOT_FOUND) */
+    MIFFT_CONV_FP16_PURE = 1,    /* .float16Pure: FP16 arithmetic and storage */
+    MIFFT_CONV_FP16_STORAGE = 2, /* .float16Storage: FP16 LDS storage, FP32 arithmetic */
+    MIFFT_CONV_FP16_MIXED = 3    /* .float16Mixed: FP16 twiddle/filter products, FP32 butterflies */
 } mifft_conv_precision;
 
 /* Convolver.init(kernel:fftSize:) (Convolve.swift:19-32) with fused = 0, or

@@ -42,8 +42,6 @@ mifft_status mifft_convolver_create(const float* taps, int64_t k, int64_t fft_si
         return fail(MIFFT_ERR_PRECONDITION, "Convolver computes in FP32 only");
     if (precision < MIFFT_CONV_FP32 || precision > MIFFT_CONV_FP16_MIXED)
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown precision");
-    if (precision != MIFFT_CONV_FP32)
-        return fail(MIFFT_ERR_KERNEL_NOT_FOUND, "HIP kernel not found: FP16 fused convolve modes");
     if (!(fft_size > k))
         return fail(MIFFT_ERR_PRECONDITION,
                     "fftSize must be > kernel.count; got fftSize=" + std::to_string(fft_size) +

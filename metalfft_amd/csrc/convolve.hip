@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 
 #include "convolve.hpp"
 #include "runtime.hpp"
@@ -52,9 +53,57 @@ struct ConvGeom {
     int64_t npair;  // block pairs per signal
 };
 
-__device__ __forceinline__ float ld_or0(const float* p, int64_t i, int64_t lo, int64_t hi) {
-    return (i >= lo && i < hi) ? p[i] : 0.f;
+// A block's window into one signal: element n (0 <= n < N) is sig[w + n]
+// when 0 <= w + n < len, else 0.  base + clo .. base + chi - 1 is always
+// valid memory (len >= 1), even for an empty window.
+struct Win {
+    const float* base;
+    int lo, hi;    // valid n
+    int clo, chi;  // clamp range for the unconditional load
+};
+
+__device__ __forceinline__ Win window(const float* sig, int64_t len, int64_t w, int n_max) {
+    int64_t lo = w < 0 ? -w : 0, hi = len - w;
+    if (hi > n_max) hi = n_max;
+    if (lo >= hi) return Win{sig, 0, 0, 0, 1};
+    return Win{sig + w, (int)lo, (int)hi, (int)lo, (int)hi};
 }
+
+// Window element n, +0 outside.  The load is unconditional, of a clamped
+// index, and masked with an opaque all-ones/zero word: a load the compiler
+// may skip on some path gets sunk under a branch and waited for at the
+// join, one HBM round trip per element, where unconditional loads all stay
+// in flight together.  32-bit offsets from the window base keep the address
+// arithmetic small.
+__device__ __forceinline__ float ld_win(const Win& W, int n) {
+    const int c = n < W.clo ? W.clo : (n >= W.chi ? W.chi - 1 : n);
+    const float v = W.base[c];
+    uint32_t keep = (n >= W.lo && n < W.hi) ? 0xffffffffu : 0u;
+    __asm__("" : "+v"(keep));
+    return __uint_as_float(__float_as_uint(v) & keep);
+}
+
+// The plan's twiddle memory holds packed rows for the radix-16 schedule
+// only (twiddle_table_len): E = 16 transforms take their table entries from
+// registers (PreBases, re-marked before each transform so derived powers
+// are not held from one to the next); E < 16 read the N-entry table.
+template <int N, int E>
+struct ConvTwiddles {
+    PreBases<N, E> pb;
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+        if constexpr (E == 16) pb.load_packed(tw, t);
+    }
+    template <bool SYNC0>
+    __device__ __forceinline__ void fft(float2 (&v)[E], float2* buf, int t,
+                                        const float2* __restrict__ tw) {
+        if constexpr (E == 16) {
+            pb.opaque();
+            stockham_regs<N, E, LdsPad, 0, SYNC0, true, false, 2>(v, buf, t, tw, pb.w);
+        } else {
+            stockham_regs<N, E, LdsPad, 0, SYNC0>(v, buf, t, tw);
+        }
+    }
+};
 
 // --------------------------------------------------------------- forward
 template <int N, int E, int TPW>
@@ -80,26 +129,27 @@ k_conv_fwd(const float* __restrict__ x, float* __restrict__ y, const float2* __r
     const bool has1 = b1 < g.nblk;
 
     float2 v[E];
+    const Win W0 = window(xs, g.S, w0, N), W1 = window(xs, g.S, w1, N);  // S >= 1
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int n = t + T * m;
-        float re = 0.f, im = 0.f;
-        if (active) {
-            re = ld_or0(xs, w0 + n, 0, g.S);
-            if (has1) im = ld_or0(xs, w1 + n, 0, g.S);
-        }
-        v[m] = make_float2(re, im);
+        const float re = ld_win(W0, n);
+        const float im = ld_win(W1, n);
+        v[m] = make_float2(active ? re : 0.f, active && has1 ? im : 0.f);
     }
-    stockham_regs<N, E, LdsPad, 0, false>(v, buf, t, tw);
+    ConvTwiddles<N, E> ct;
+    ct.load(tw, t);
+    ct.template fft<false>(v, buf, t, tw);
     // multiply by the filter spectrum, then inverse via conj-FFT-conj
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], H[t + T * m]));
-    stockham_regs<N, E, LdsPad, 0, true>(v, buf, t, tw);
+    ct.template fft<true>(v, buf, t, tw);
     if (!active) return;
     constexpr float inv_n = 1.0f / (float)N;
+    const int ts = fresh(t);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        const int n = t + T * m;
+        const int n = ts + T * m;
         const int j = n - (g.K - 1);
         if (j >= 0 && j < g.hop) {
             const int64_t o0 = b0 * g.hop + j;
@@ -111,11 +161,75 @@ k_conv_fwd(const float* __restrict__ x, float* __restrict__ y, const float2* __r
 }
 
 // --------------------------------------------------------------- adjoint
+// Two kernels: grad_x (two transforms per block pair, like the forward) and
+// grad_h (two forward transforms per pair, spectra accumulated per slot,
+// one inverse per slot).  One fused kernel needs both pair spectra, the
+// accumulator and a third transform live at once, which costs it most of
+// its occupancy; the split spends one more transform per pair instead.
+
+// Block pair q of signal s: z = two gradient windows gw_b[n] = gy[b*hop + n].
+template <int N, int E>
+__device__ __forceinline__ void load_grad_pair(float2 (&z)[E], const float* gs, int64_t ylen,
+                                               int64_t p0, int64_t p1, bool active, bool a1, int t) {
+    constexpr int T = N / E;
+    const Win G0 = window(gs, ylen, p0, N), G1 = window(gs, ylen, p1, N);
+    const int tz = fresh(t);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int n = tz + T * m;
+        const float zr = ld_win(G0, n);
+        const float zi = ld_win(G1, n);
+        z[m] = make_float2(active ? zr : 0.f, a1 ? zi : 0.f);
+    }
+}
+
+// grad_x[b*hop + j] = IFFT(GW_b * conj(H))[j], j < hop: one block pair per
+// transform slot.
 template <int N, int E, int TPW>
 __global__ void __launch_bounds__((N / E) * TPW)
-k_conv_bwd(const float* __restrict__ x, const float* __restrict__ gy,
-           float* __restrict__ gx, float* __restrict__ gh_part, const float2* __restrict__ H,
-           const float2* __restrict__ tw, ConvGeom g, int64_t nbx, int64_t npx, int ppw) {
+k_conv_bwd_x(const float* __restrict__ gy, float* __restrict__ gx, const float2* __restrict__ H,
+             const float2* __restrict__ tw, ConvGeom g, int64_t nbx, int64_t npx) {
+    constexpr int T = N / E;
+    __shared__ float2 lds[TPW * LdsPad::len(N)];
+    const int local = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    float2* buf = lds + local * LdsPad::len(N);
+    const int64_t item = (int64_t)blockIdx.x * TPW + local;
+    const bool active = item < g.count * npx;
+    const int64_t s = active ? item / npx : 0;
+    const int64_t q = active ? item - s * npx : 0;
+    const int64_t ylen = g.S + g.K - 1;
+    const int64_t p0 = 2 * q * g.hop, p1 = p0 + g.hop;
+    const bool has1 = 2 * q + 1 < nbx;
+    float2 z[E];
+    load_grad_pair<N, E>(z, gy + s * ylen, ylen, p0, p1, active, active && has1, t);
+    ConvTwiddles<N, E> ct;
+    ct.load(tw, t);
+    ct.template fft<false>(z, buf, t, tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) z[m] = conjf2(cmulc(z[m], H[t + T * m]));  // conj(Z conj(H))
+    ct.template fft<true>(z, buf, t, tw);
+    if (!active) return;
+    constexpr float inv_n = 1.0f / (float)N;
+    float* gxs = gx + s * g.S;
+    const int tg = fresh(t);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int n = tg + T * m;
+        if (n < g.hop) {
+            if (p0 + n < g.S) gxs[p0 + n] = z[m].x * inv_n;
+            if (has1 && p1 + n < g.S) gxs[p1 + n] = -z[m].y * inv_n;
+        }
+    }
+}
+
+// Partial tap gradient of one slot (ppw block pairs): Re IFFT(sum conj(U) Z)[k],
+// k < K, with U the spectra of the pairs' signal blocks (hop samples each).
+template <int N, int E, int TPW>
+__global__ void __launch_bounds__((N / E) * TPW)
+k_conv_bwd_h(const float* __restrict__ x, const float* __restrict__ gy,
+             float* __restrict__ gh_part, const float2* __restrict__ tw, ConvGeom g, int64_t nbx,
+             int64_t npx, int ppw) {
     constexpr int T = N / E;
     __shared__ float2 lds[TPW * LdsPad::len(N)];
     const int local = threadIdx.x / T;
@@ -127,59 +241,43 @@ k_conv_bwd(const float* __restrict__ x, const float* __restrict__ gy,
     float2 acc[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) acc[m] = make_float2(0.f, 0.f);
+    ConvTwiddles<N, E> ct;
+    ct.load(tw, t);
 
     for (int p = 0; p < ppw; ++p) {  // uniform trip count across the workgroup
         const int64_t item = ((int64_t)blockIdx.x * ppw + p) * TPW + local;
         const bool active = item < total;
         const int64_t s = active ? item / npx : 0;
-        const int64_t q = active ? item % npx : 0;
-        const int64_t b0 = 2 * q, b1 = 2 * q + 1;
-        const bool has1 = b1 < nbx;
-        const float* xs = x + s * g.S;
-        const float* gs = gy + s * ylen;
-        const int64_t p0 = b0 * g.hop, p1 = b1 * g.hop;
-
+        const int64_t q = active ? item - s * npx : 0;
+        const int64_t p0 = 2 * q * g.hop, p1 = p0 + g.hop;
+        const bool a1 = active && 2 * q + 1 < nbx;
+        // opaque copies of t per iteration: LDS and global index arithmetic
+        // is recomputed each time instead of hoisted out of the loop and
+        // held live through it
+        const int tl = fresh(t);
         float2 z[E], u[E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int n = t + T * m;
-            float zr = 0.f, zi = 0.f, ur = 0.f, ui = 0.f;
-            if (active) {
-                zr = ld_or0(gs, p0 + n, 0, ylen);
-                if (n < g.hop) ur = ld_or0(xs, p0 + n, 0, g.S);
-                if (has1) {
-                    zi = ld_or0(gs, p1 + n, 0, ylen);
-                    if (n < g.hop) ui = ld_or0(xs, p1 + n, 0, g.S);
-                }
-            }
-            z[m] = make_float2(zr, zi);
-            u[m] = make_float2(ur, ui);
-        }
-        stockham_regs<N, E, LdsPad, 0, true>(z, buf, t, tw);
-        stockham_regs<N, E, LdsPad, 0, true>(u, buf, t, tw);
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            acc[m] = cadd(acc[m], cmulc(z[m], u[m]));  // conj(U) Z
-            z[m] = conjf2(cmulc(z[m], H[t + T * m]));  // conj(Z conj(H)) for the inverse
-        }
-        stockham_regs<N, E, LdsPad, 0, true>(z, buf, t, tw);
-        if (active) {
-            constexpr float inv_n = 1.0f / (float)N;
-            float* gxs = gx + s * g.S;
+        load_grad_pair<N, E>(z, gy + s * ylen, ylen, p0, p1, active, a1, tl);
+        ct.template fft<true>(z, buf, tl, tw);
+        __builtin_amdgcn_sched_barrier(0);  // the signal loads stay below the transform
+        {
+            const float* xs = x + s * g.S;
+            const Win X0 = window(xs, g.S, p0, g.hop), X1 = window(xs, g.S, p1, g.hop);
+            const int tu = fresh(t);
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                const int n = t + T * m;
-                if (n < g.hop) {
-                    if (p0 + n < g.S) gxs[p0 + n] = z[m].x * inv_n;
-                    if (has1 && p1 + n < g.S) gxs[p1 + n] = -z[m].y * inv_n;
-                }
+                const int n = tu + T * m;
+                const float ur = ld_win(X0, n);
+                const float ui = ld_win(X1, n);
+                u[m] = make_float2(active ? ur : 0.f, a1 ? ui : 0.f);
             }
         }
+        ct.template fft<true>(u, buf, fresh(t), tw);
+#pragma unroll
+        for (int m = 0; m < E; ++m) acc[m] = cadd(acc[m], cmulc(z[m], u[m]));  // conj(U) Z
     }
-    // partial tap gradient of this slot: Re IFFT(acc)[k], k < K
 #pragma unroll
     for (int m = 0; m < E; ++m) acc[m] = conjf2(acc[m]);
-    stockham_regs<N, E, LdsPad, 0, true>(acc, buf, t, tw);
+    ct.template fft<true>(acc, buf, t, tw);
     constexpr float inv_n = 1.0f / (float)N;
     float* part = gh_part + ((int64_t)blockIdx.x * TPW + local) * g.K;
 #pragma unroll
@@ -197,6 +295,223 @@ __global__ void k_sum_columns(const float* __restrict__ part, float* __restrict_
     float s = 0.f;
     for (int64_t r = 0; r < rows; ++r) s += part[r * cols + k];
     out[k] = s;
+}
+
+// --------------------------------------------------- FP16 precision modes
+// FusedConvolver.Precision (Sources/MetalFFT/DSP/FusedConvolver.swift:24-35,
+// kernels Sources/MetalFFT/Resources/fft_fused_convolve_fp16.metal:160-498),
+// N = 4096 only as in the reference:
+//   PURE     butterflies, twiddle powers, twiddle and filter products and the
+//            1/N scale in FP16 (packed f16 VALU), FP16 LDS exchange;
+//   STORAGE  every LDS exchange stored as FP16, all arithmetic in FP32;
+//   MIXED    twiddle powers, twiddle and filter products in FP16,
+//            butterflies in FP32, FP16 LDS exchange.
+// Where the reference keeps data in threadgroup memory (between passes, and
+// the spectrum around the filter product) the value is FP16 here too; its
+// radix-4 passes are radix-16 here (3 roundings per transform instead of 6).
+// Same overlap-save geometry as k_conv_fwd; the half2 exchange buffer is
+// half the FP32 one.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+enum { kConvPure = 1, kConvStorage = 2, kConvMixed = 3 };
+
+__device__ __forceinline__ h2 to_h2(float2 a) { return (h2){(_Float16)a.x, (_Float16)a.y}; }
+__device__ __forceinline__ float2 to_f2(h2 a) { return make_float2((float)a.x, (float)a.y); }
+__device__ __forceinline__ float2 round_h(float2 a) { return to_f2(to_h2(a)); }
+__device__ __forceinline__ h2 hmul(h2 a, h2 b) {
+    return (h2){a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ h2 hmul_mi(h2 a) { return (h2){a.y, -a.x}; }
+__device__ __forceinline__ h2 hconj(h2 a) { return (h2){a.x, -a.y}; }
+
+__device__ __forceinline__ void hdft4(h2& x0, h2& x1, h2& x2, h2& x3) {
+    const h2 t0 = x0 + x2, t1 = x1 + x3, t2 = x0 - x2, t3 = hmul_mi(x1 - x3);
+    x0 = t0 + t1;
+    x2 = t0 - t1;
+    x1 = t2 + t3;
+    x3 = t2 - t3;
+}
+
+// DFT-16 in FP16: the 4 x 4 split of dft16 (fft_device.hpp), every operation
+// rounded to FP16.
+__device__ __forceinline__ void hdft16(h2* x) {
+    h2 a[4][4];
+#pragma unroll
+    for (int j2 = 0; j2 < 4; ++j2) {
+        h2 p0 = x[j2], p1 = x[4 + j2], p2 = x[8 + j2], p3 = x[12 + j2];
+        hdft4(p0, p1, p2, p3);
+        a[j2][0] = p0; a[j2][1] = p1; a[j2][2] = p2; a[j2][3] = p3;
+    }
+    const _Float16 c = (_Float16)MIFFT_C16, sn = (_Float16)MIFFT_S16, h = (_Float16)MIFFT_SQRT1_2;
+    const h2 w1 = (h2){c, -sn}, w3 = (h2){sn, -c}, w9 = (h2){-c, sn};
+    auto w2 = [&](h2 v) { return (h2){h * (v.x + v.y), h * (v.y - v.x)}; };
+    auto w6 = [&](h2 v) { return (h2){h * (v.y - v.x), -h * (v.x + v.y)}; };
+    a[1][1] = hmul(a[1][1], w1);
+    a[1][2] = w2(a[1][2]);
+    a[1][3] = hmul(a[1][3], w3);
+    a[2][1] = w2(a[2][1]);
+    a[2][2] = hmul_mi(a[2][2]);
+    a[2][3] = w6(a[2][3]);
+    a[3][1] = hmul(a[3][1], w3);
+    a[3][2] = w6(a[3][2]);
+    a[3][3] = hmul(a[3][3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        h2 q0 = a[0][k1], q1 = a[1][k1], q2 = a[2][k1], q3 = a[3][k1];
+        hdft4(q0, q1, q2, q3);
+        x[k1] = q0; x[k1 + 4] = q1; x[k1 + 8] = q2; x[k1 + 12] = q3;
+    }
+}
+
+// w[j] = W^{jk}, j < 16, from the FP32 bases W^k, W^2k, W^4k, W^8k, the
+// products in FP16 (the reference forms w2 = w1*w1, w3 = w2*w1 in half).
+__device__ __forceinline__ void hpowers16(h2 (&w)[16], const float2* b) {
+    w[0] = (h2){(_Float16)1.f, (_Float16)0.f};
+    w[1] = to_h2(b[0]);
+    w[2] = to_h2(b[1]);
+    w[3] = hmul(w[1], w[2]);
+    w[4] = to_h2(b[2]);
+    w[5] = hmul(w[1], w[4]);
+    w[6] = hmul(w[2], w[4]);
+    w[7] = hmul(w[3], w[4]);
+    w[8] = to_h2(b[3]);
+#pragma unroll
+    for (int j = 9; j < 16; ++j) w[j] = hmul(w[j - 8], w[8]);
+}
+
+template <int PM>
+using ConvReg = typename std::conditional<PM == kConvPure, h2, float2>::type;
+
+// Pass P of the N = 4096 radix-16 Stockham transform (stockham_regs) with
+// the mode's arithmetic; pb: the thread's FP32 table entries (PreBases).
+template <int PM, int P, bool SYNC0>
+__device__ __forceinline__ void conv_passes_h(ConvReg<PM> (&v)[16], h2* buf, int t,
+                                              const float2 (*pb)[16]) {
+    using S = Schedule<4096, 16>;
+    constexpr int SD = S::stride(P);
+    constexpr int T = S::T;
+    if constexpr (SD > 1) {
+        if constexpr (PM == kConvStorage) {
+            float2 w[16];
+            powers_from_bases<16>(w, pb[P - 1]);
+#pragma unroll
+            for (int j = 1; j < 16; ++j) v[j] = cmul(v[j], w[j]);
+        } else {
+            h2 w[16];
+            hpowers16(w, pb[P - 1]);
+#pragma unroll
+            for (int j = 1; j < 16; ++j) {
+                if constexpr (PM == kConvPure) v[j] = hmul(v[j], w[j]);
+                else v[j] = to_f2(hmul(to_h2(v[j]), w[j]));
+            }
+        }
+    }
+    if constexpr (PM == kConvPure) hdft16(v); else dft<16>(v);
+    if constexpr (P < S::passes - 1) {
+        if constexpr (P > 0 || SYNC0) lds_barrier();
+        const int base = (t / SD) * (16 * SD) + (t & (SD - 1));
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if constexpr (PM == kConvPure) buf[LdsPad::idx(base + j * SD)] = v[j];
+            else buf[LdsPad::idx(base + j * SD)] = to_h2(v[j]);
+        }
+        lds_barrier();
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            if constexpr (PM == kConvPure) v[m] = buf[LdsPad::idx(t + T * m)];
+            else v[m] = to_f2(buf[LdsPad::idx(t + T * m)]);
+        }
+        conv_passes_h<PM, P + 1, SYNC0>(v, buf, t, pb);
+    }
+}
+
+template <int PM>
+__global__ void __launch_bounds__(256)
+k_conv_fwd_h(const float* __restrict__ x, float* __restrict__ y, const float2* __restrict__ H,
+             const float2* __restrict__ tw, ConvGeom g) {
+    constexpr int N = 4096, T = 256;
+    __shared__ h2 lds[LdsPad::len(N)];
+    const int t = threadIdx.x;
+    const int64_t item = blockIdx.x;  // one block pair per workgroup; grid == count * npair
+    const int64_t s = item / g.npair;
+    const int64_t q = item % g.npair;
+    const int64_t b0 = 2 * q, b1 = 2 * q + 1;
+    const float* xs = x + s * g.S;
+    const int64_t ylen = g.S + g.K - 1;
+    float* ys = y + s * ylen;
+    const int64_t w0 = b0 * g.hop - (g.K - 1);
+    const int64_t w1 = b1 * g.hop - (g.K - 1);
+    const bool has1 = b1 < g.nblk;
+
+    ConvReg<PM> v[16];
+    const Win W0 = window(xs, g.S, w0, N), W1 = window(xs, g.S, w1, N);  // S >= 1
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int n = t + T * m;
+        const float re = ld_win(W0, n);
+        const float l1 = ld_win(W1, n);
+        const float im = has1 ? l1 : 0.f;
+        if constexpr (PM == kConvPure) v[m] = to_h2(make_float2(re, im));
+        else v[m] = make_float2(re, im);
+    }
+    PreBases<N, 16> pb;
+    pb.load_packed(tw, t);
+    conv_passes_h<PM, 0, false>(v, lds, t, pb.w);
+    // spectrum (FP16, as it sits in threadgroup memory) times H, conjugated
+    // for the inverse transform
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const float2 hm = H[t + T * m];
+        if constexpr (PM == kConvPure) v[m] = hconj(hmul(v[m], to_h2(hm)));
+        else if constexpr (PM == kConvMixed) v[m] = to_f2(hconj(hmul(to_h2(v[m]), to_h2(hm))));
+        else v[m] = conjf2(round_h(cmul(round_h(v[m]), hm)));
+    }
+    pb.opaque();
+    conv_passes_h<PM, 0, true>(v, lds, t, pb.w);
+    const int ts = fresh(t);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int n = ts + T * m;
+        const int j = n - (g.K - 1);
+        if (j >= 0 && j < g.hop) {
+            float2 r;  // conj(.) / N
+            if constexpr (PM == kConvPure) {
+                const _Float16 inv_n = (_Float16)(1.0f / (float)N);
+                r = to_f2((h2){v[m].x * inv_n, -v[m].y * inv_n});
+            } else {
+                constexpr float inv_n = 1.0f / (float)N;
+                r = make_float2(v[m].x * inv_n, -v[m].y * inv_n);
+            }
+            const int64_t o0 = b0 * g.hop + j;
+            if (o0 < ylen) ys[o0] = r.x;
+            const int64_t o1 = b1 * g.hop + j;
+            if (has1 && o1 < ylen) ys[o1] = r.y;
+        }
+    }
+}
+
+hipError_t conv_fwd_fp16(int precision, const float* x, float* y, const float2* H, const float2* tw,
+                         ConvGeom g, hipStream_t st) {
+    const int64_t blocks = g.count * g.npair;
+    if (blocks <= 0) return hipSuccess;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (precision) {
+        case kConvPure:
+            hipLaunchKernelGGL(k_conv_fwd_h<kConvPure>, dim3((unsigned)blocks), dim3(256), 0, st, x, y,
+                               H, tw, g);
+            break;
+        case kConvStorage:
+            hipLaunchKernelGGL(k_conv_fwd_h<kConvStorage>, dim3((unsigned)blocks), dim3(256), 0, st, x,
+                               y, H, tw, g);
+            break;
+        case kConvMixed:
+            hipLaunchKernelGGL(k_conv_fwd_h<kConvMixed>, dim3((unsigned)blocks), dim3(256), 0, st, x, y,
+                               H, tw, g);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 __global__ void k_zero(float* __restrict__ p, int64_t n) {
@@ -235,10 +550,14 @@ hipError_t conv_bwd_n(const float* x, const float* gy, float* gx, float* part, i
                       const float2* H, const float2* tw, ConvGeom g, int64_t nbx, int64_t npx,
                       int ppw, hipStream_t st) {
     using C = ConvCfg<N>;
-    const int64_t blocks = slots / C::TPW;
-    if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_conv_bwd<N, C::E, C::TPW>), dim3((unsigned)blocks), dim3(C::threads), 0, st,
-                       x, gy, gx, part, H, tw, g, nbx, npx, ppw);
+    const int64_t xblocks = (g.count * npx + C::TPW - 1) / C::TPW;
+    const int64_t hblocks = slots / C::TPW;
+    if (xblocks <= 0 || hblocks <= 0) return hipSuccess;
+    if (xblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_conv_bwd_x<N, C::E, C::TPW>), dim3((unsigned)xblocks), dim3(C::threads), 0,
+                       st, gy, gx, H, tw, g, nbx, npx);
+    hipLaunchKernelGGL((k_conv_bwd_h<N, C::E, C::TPW>), dim3((unsigned)hblocks), dim3(C::threads), 0,
+                       st, x, gy, part, tw, g, nbx, npx, ppw);
     return hipGetLastError();
 }
 
@@ -297,18 +616,18 @@ k_correlate(const float* __restrict__ a, int64_t na, const float* __restrict__ b
     const bool active = item < count;
     const float* as = a + (active ? item : 0) * na;
     const float* bs = b + (active ? item : 0) * nb;
+    const Win A = window(as, na > 0 ? na : 1, 0, N), Bw = window(bs, nb > 0 ? nb : 1, 0, N);
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
+    for (int m = 0; m < E; ++m) {  // branch-free loads (ld_win); an empty input reads as 0
         const int n = t + T * m;
-        float re = 0.f, im = 0.f;
-        if (active) {
-            if (n < na) re = as[n];
-            if (n < nb) im = bs[n];
-        }
-        v[m] = make_float2(re, im);
+        const float re = na > 0 ? ld_win(A, n) : 0.f;
+        const float im = nb > 0 ? ld_win(Bw, n) : 0.f;
+        v[m] = make_float2(active ? re : 0.f, active ? im : 0.f);
     }
-    stockham_regs<N, E, LdsPad, 0, false>(v, buf, t, tw);
+    ConvTwiddles<N, E> ct;
+    ct.load(tw, t);
+    ct.template fft<false>(v, buf, t, tw);
     lds_barrier();
 #pragma unroll
     for (int m = 0; m < E; ++m) buf[LdsPad::idx(t + T * m)] = v[m];
@@ -324,7 +643,7 @@ k_correlate(const float* __restrict__ a, int64_t na, const float* __restrict__ b
         const float2 P = make_float2(A.x * B.x + A.y * B.y, A.y * B.x - A.x * B.y);
         v[m] = conjf2(P);
     }
-    stockham_regs<N, E, LdsPad, 0, true>(v, buf, t, tw);
+    ct.template fft<true>(v, buf, t, tw);
     if (!active) return;
     constexpr float inv_n = 1.0f / (float)N;
     float* rs = r + item * N;
@@ -378,7 +697,14 @@ mifft_status convolver_spectrum(mifft_convolver* cv, const float* taps_dev, hipS
 
 mifft_status convolver_forward(const mifft_convolver* cv, const float* x, int64_t S, int64_t count,
                                float* y, hipStream_t st) {
+    if (S == 0) return zero_floats(y, count * (cv->K - 1), st);  // y = x * h is all zeros
     ConvGeom g = geometry(cv, S, count);
+    if (cv->precision != MIFFT_CONV_FP32) {
+        if (cv->n != 4096) return fail(MIFFT_ERR_KERNEL_NOT_FOUND, "FP16 fused convolve is n = 4096 only");
+        return check_hip(conv_fwd_fp16(cv->precision, x, y, static_cast<const float2*>(cv->spectrum.ptr),
+                                       cv->plan->twiddles, g, st),
+                         "fused convolve (FP16)");
+    }
     return check_hip(conv_fwd((int)cv->n, x, y, static_cast<const float2*>(cv->spectrum.ptr),
                               cv->plan->twiddles, g, st),
                      "fused convolve");

@@ -57,6 +57,15 @@ __device__ __forceinline__ void ds_barrier() {
     __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// An opaque copy of an index (empty asm): arithmetic derived from it in one
+// phase or loop iteration (LDS and global addresses) is recomputed there,
+// instead of being shared with another phase or hoisted out of a loop and
+// held live in registers through everything in between.
+__device__ __forceinline__ int fresh(int i) {
+    __asm__("" : "+v"(i));
+    return i;
+}
+
 // ------------------------------------------------------- global I/O helpers
 typedef float f2v __attribute__((ext_vector_type(2)));
 
@@ -355,6 +364,20 @@ struct PreBases {
             load_packed<p + 1>(tw, t);
         }
     }
+    // Marks the entries as rewritten (empty asm).  Between two transforms
+    // of one thread this stops the compiler from keeping the first
+    // transform's derived powers (R - 1 per pass) live for reuse by the
+    // second; they are cheaper to derive again than to hold.
+    template <int p = 1>
+    __device__ __forceinline__ void opaque() {
+        if constexpr (p <= NP) {
+            constexpr int r = S::radix(p), b = E / r, nb = twiddle_bases(r);
+#pragma unroll
+            for (int i = 0; i < b * nb; ++i)
+                __asm__ volatile("" : "+v"(w[p - 1][i].x), "+v"(w[p - 1][i].y));
+            opaque<p + 1>();
+        }
+    }
 };
 
 // One launch = ceil(batch / TPW) workgroups, TPW transforms each.  `in` and
@@ -414,17 +437,20 @@ k_stockham_stream(const float2* in, float2* out,
     // the loop bound depends on blockIdx only: every thread of a workgroup
     // runs the same iterations, so barriers stay uniform
     for (int64_t base = (int64_t)blockIdx.x * TPW; base < batch; base += step) {
+        const int tl = fresh(t);  // per-iteration index arithmetic (not hoisted)
         tr = base + local;
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         const int64_t trn = tr + step;
-        load_column<N, E, INV, NT>(nxt, in + trn * N, t, trn < batch);
-        if constexpr (TWM == 1 || TWM == 2)
-            stockham_regs<N, E, L, 0, true, TW4, false, 2>(v, buf, t, tw, pb.w);
-        else
-            stockham_regs<N, E, L, 0, true, TW4, false, TWM>(v, buf, t, tw);
-        if (tr < batch) store_column<N, E, INV, NT>(v, out + tr * N, t);
+        load_column<N, E, INV, NT>(nxt, in + trn * N, tl, trn < batch);
+        if constexpr (TWM == 1 || TWM == 2) {
+            pb.opaque();
+            stockham_regs<N, E, L, 0, true, TW4, false, 2>(v, buf, tl, tw, pb.w);
+        } else {
+            stockham_regs<N, E, L, 0, true, TW4, false, TWM>(v, buf, tl, tw);
+        }
+        if (tr < batch) store_column<N, E, INV, NT>(v, out + tr * N, fresh(t));
     }
 }
 

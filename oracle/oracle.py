@@ -144,6 +144,112 @@ def f64_correlate(a: np.ndarray, b: np.ndarray, fft_size: int) -> np.ndarray:
     return np.real(np.fft.ifft(np.fft.fft(pa) * np.conj(np.fft.fft(pb))))
 
 
+# ------------------------------------------- FusedConvolver FP16 emulation
+# The reference's FP16 fused convolution (Sources/MetalFFT/Resources/
+# fft_fused_convolve_fp16.metal:160-498) restated in numpy, step by step:
+# 6 radix-4 Stockham passes (tid < 1024 reads buf[tid + j*1024]; pass p > 0
+# twiddles W^{pos*1024/stride} from FP32 sincos with w2 = w1*w1, w3 = w2*w1;
+# writes grp*4*stride + pos + j*stride), threadgroup storage in FP16 after
+# every pass, filter product, conj-FFT-conj inverse, 1/N scale; host
+# overlap-add as FusedConvolver.apply (FusedConvolver.swift:91-133).  numpy
+# float16 arithmetic rounds every operation to FP16 (the Apple GPU may fuse
+# multiply-adds), so this pins the error magnitude of each mode, not bits.
+_H = np.float16
+_F = np.float32
+
+
+def _cmul(ar, ai, br, bi):
+    return ar * br - ai * bi, ar * bi + ai * br
+
+
+def _fp16_pass(xr, xi, p, mode, _H=_H, _F=_F):
+    """Pass p of the N = 4096 radix-4 transform on (B, 4096) arrays holding
+    the threadgroup buffer (FP16 for every mode); returns the new buffer."""
+    n, q = 4096, 1024
+    stride = 4 ** p
+    tid = np.arange(q)
+    pos, grp = tid & (stride - 1), tid // stride
+    ar = [xr[:, j * q:(j + 1) * q] for j in range(4)]
+    ai = [xi[:, j * q:(j + 1) * q] for j in range(4)]
+    if mode != "pure":
+        ar = [a.astype(_F) for a in ar]
+        ai = [a.astype(_F) for a in ai]
+    if p > 0:
+        ang = _F(-2.0 * np.pi / n) * (pos * (q // stride)).astype(_F)
+        c1, s1 = np.cos(ang).astype(_F), np.sin(ang).astype(_F)
+        if mode == "storage":
+            w1 = (c1, s1)
+        else:
+            w1 = (c1.astype(_H), s1.astype(_H))
+        w2 = _cmul(*w1, *w1)
+        w3 = _cmul(*w2, *w1)
+        for j, w in ((1, w1), (2, w2), (3, w3)):
+            if mode == "mixed":
+                r, i = _cmul(ar[j].astype(_H), ai[j].astype(_H), *w)
+                ar[j], ai[j] = r.astype(_F), i.astype(_F)
+            else:
+                ar[j], ai[j] = _cmul(ar[j], ai[j], *w)
+    t0 = (ar[0] + ar[2], ai[0] + ai[2])
+    t1 = (ar[1] + ar[3], ai[1] + ai[3])
+    t2 = (ar[0] - ar[2], ai[0] - ai[2])
+    t3 = (ai[1] - ai[3], -(ar[1] - ar[3]))  # (x1 - x3) * -i
+    y = [(t0[0] + t1[0], t0[1] + t1[1]), (t2[0] + t3[0], t2[1] + t3[1]),
+         (t0[0] - t1[0], t0[1] - t1[1]), (t2[0] - t3[0], t2[1] - t3[1])]
+    wr = grp * 4 * stride + pos
+    outr = np.empty((xr.shape[0], n), dtype=y[0][0].dtype)
+    outi = np.empty_like(outr)
+    for j in range(4):
+        outr[:, wr + j * stride] = y[j][0]
+        outi[:, wr + j * stride] = y[j][1]
+    return outr, outi
+
+
+def fused_convolve_fp16_emulated(signal: np.ndarray, taps: np.ndarray, mode: str,
+                                 _H=_H, _F=_F) -> np.ndarray:
+    """FusedConvolver(kernel: taps, fftSize: 4096, precision: .float16<mode>)
+    .apply(to: signal), mode in {"pure", "storage", "mixed"}.  (_H, _F
+    replace the FP16 / FP32 types; with float64 for both the steps must give
+    the exact linear convolution, which the tests check.)"""
+    n = 4096
+    x = np.asarray(signal, np.float32)
+    k = np.asarray(taps, np.float32)
+    hop = n - k.size + 1
+    spec = np.fft.fft(np.pad(k.astype(np.float64), (0, n - k.size))).astype(np.complex64)
+    nblk = (x.size + hop - 1) // hop
+    blocks = np.zeros((nblk, n), np.float32)
+    for b in range(nblk):
+        seg = x[b * hop:(b + 1) * hop]
+        blocks[b, :seg.size] = seg
+    # input load: half2(src) for pure, FP32 otherwise; pass 0 reads it
+    if mode == "pure":
+        br, bi = blocks.astype(_H), np.zeros_like(blocks, dtype=_H)
+    else:
+        br, bi = blocks, np.zeros_like(blocks)
+    for p in range(6):
+        br, bi = _fp16_pass(br, bi, p, mode, _H, _F)
+        br, bi = br.astype(_H), bi.astype(_H)  # threadgroup storage
+    fr, fi = spec.real.astype(_F), spec.imag.astype(_F)
+    if mode == "storage":
+        r, i = _cmul(br.astype(_F), bi.astype(_F), fr, fi)
+    else:
+        r, i = _cmul(br, bi, fr.astype(_H), fi.astype(_H))
+    br, bi = r.astype(_H), (-i).astype(_H)  # stored, then conjugated in place
+    for p in range(6):
+        br, bi = _fp16_pass(br, bi, p, mode, _H, _F)
+        if p < 5:
+            br, bi = br.astype(_H), bi.astype(_H)
+    if mode == "pure":
+        yb = (br.astype(_H) * _H(1.0 / n)).astype(np.float64)
+    else:
+        yb = (br.astype(_F) * _F(1.0 / n)).astype(np.float64)
+    out = np.zeros(x.size + k.size - 1)
+    for b in range(nblk):
+        lo = b * hop
+        hi = min(lo + n, out.size)
+        out[lo:hi] += yb[b, :hi - lo]
+    return out
+
+
 # ---------------------------------------------------------------- metrics
 def rel_l2(a, b) -> float:
     """sqrt(sum|a-b|^2 / sum|b|^2), Tests/MetalFFTTests/MetalFFTTests.swift:190-198."""

@@ -248,7 +248,55 @@ def test_correlator_lag_convention(gpu):
     assert int(np.argmax(r)) == 37
 
 
-def test_fp16_modes_not_built(gpu):
+FP16_MODES = [("float16Pure", "pure", 2e-2), ("float16Storage", "storage", 1e-3),
+              ("float16Mixed", "mixed", 2e-2)]
+
+
+@pytest.mark.parametrize("name,mode,thr", FP16_MODES)
+def test_fp16_modes_vs_fp32_and_reference_emulation(gpu, oracle, name, mode, thr):
+    # FusedConvolverTests.swift:39-58: 64 taps, fftSize + 500 samples, the
+    # FP16 result within `thr` relative L2 of the FP32 fused one; and no
+    # worse than twice the error of the reference's own FP16 algorithm
+    # (oracle emulation, numpy float16) against FP64
+    rng = np.random.default_rng(11)
+    taps = rng.uniform(-1, 1, 64).astype(np.float32)
+    sig = rng.uniform(-1, 1, 4096 + 500).astype(np.float32)
+    fp32 = gpu.FusedConvolver(taps, 4096).apply(sig)
+    cv = gpu.FusedConvolver(taps, 4096, getattr(gpu.Precision, name))
+    out = cv.apply(sig)
+    assert out.size == sig.size + 63
+    assert np.all(np.isfinite(out))
+    assert oracle.rel_l2(out, fp32) < thr
+    ref64 = oracle.f64_convolve(sig, taps)
+    emu = oracle.fused_convolve_fp16_emulated(sig, taps, mode)
+    err, err_ref = oracle.rel_l2(out, ref64), oracle.rel_l2(emu, ref64)
+    assert err < 2 * err_ref, (err, err_ref)
+    assert err > 1e-6  # FP16 really was used
+
+
+@pytest.mark.parametrize("name,mode,thr", FP16_MODES)
+def test_fp16_modes_batched_device(gpu, oracle, name, mode, thr):
+    import torch
+    rng = np.random.default_rng(12)
+    taps = (rng.standard_normal(1023) / np.sqrt(1023)).astype(np.float32)
+    x = rng.standard_normal((8, 20000)).astype(np.float32)
+    cv = gpu.FusedConvolver(taps, 4096, getattr(gpu.Precision, name))
+    xs = torch.from_numpy(x).cuda()
+    ys = torch.empty((8, 20000 + 1022), dtype=torch.float32, device="cuda")
+    cv.apply_device(xs, ys, 20000, 8)
+    torch.cuda.synchronize()
+    y = ys.cpu().numpy()
+    for r in range(8):
+        ref64 = oracle.f64_convolve(x[r], taps)
+        emu = oracle.fused_convolve_fp16_emulated(x[r], taps, mode)
+        assert oracle.rel_l2(y[r], ref64) < 2 * oracle.rel_l2(emu, ref64)
+
+
+def test_fp16_adjoint_not_built(gpu):
+    import torch
+    cv = gpu.FusedConvolver(np.ones(8, np.float32), 4096, gpu.Precision.float16Storage)
+    x = torch.zeros((1, 100), device="cuda")
     with pytest.raises(gpu.FFTError) as ei:
-        gpu.FusedConvolver(np.ones(8, np.float32), 4096, gpu.Precision.float16Storage)
+        cv.backward_device(x, torch.zeros((1, 107), device="cuda"), torch.zeros_like(x),
+                           torch.zeros(8, device="cuda"), 100, 1)
     assert ei.value.case == "kernelNotFound"

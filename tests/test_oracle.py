@@ -181,3 +181,42 @@ def test_conv_gradients_brute_force(oracle):
     f = lambda hv: sum(np.dot(np.convolve(x[s], hv), g[s]) for s in range(2))
     e = np.zeros(7); e[3] = 1e-6
     assert abs((f(h + e) - f(h - e)) / 2e-6 - gh[3]) < 1e-6
+
+
+# ----------------------------------------- FusedConvolver FP16 emulation
+def _fp16_case():
+    rng = np.random.default_rng(5)
+    return (rng.uniform(-1, 1, 4096 + 500).astype(np.float32),
+            rng.uniform(-1, 1, 64).astype(np.float32))
+
+
+def test_fp16_emulation_steps_exact_in_fp64(oracle):
+    """With FP64 in place of FP16 / FP32 the emulated reference steps
+    (radix-4 passes, twiddles, filter product, conj-FFT-conj, overlap-add)
+    give the linear convolution: a wrong index, sign or twiddle fails."""
+    x, k = _fp16_case()
+    ref = oracle.f64_convolve(x, k)
+    for mode in ("pure", "storage", "mixed"):
+        y = oracle.fused_convolve_fp16_emulated(x, k, mode, _H=np.float64, _F=np.float64)
+        assert oracle.rel_l2(y, ref) < 1e-6, mode
+
+
+@pytest.mark.parametrize("mode,thr", [("pure", 2e-2), ("storage", 1e-3), ("mixed", 2e-2)])
+def test_fp16_emulation_error_magnitude(oracle, mode, thr):
+    """FP16 rounding is really applied (error well above FP32's) and stays
+    inside the reference's own test thresholds against the FP32 path
+    (Tests/MetalFFTTests/FusedConvolverTests.swift:39-49)."""
+    x, k = _fp16_case()
+    y = oracle.fused_convolve_fp16_emulated(x, k, mode)
+    err = oracle.rel_l2(y, oracle.f64_convolve(x, k))
+    assert 1e-4 < err < thr
+    assert oracle.rel_l2(y, oracle.cpu_convolve(x, k, 4096)) < thr
+
+
+def test_fp16_emulation_mode_ordering(oracle):
+    """FP32 arithmetic with FP16 storage is the most accurate mode."""
+    x, k = _fp16_case()
+    ref = oracle.f64_convolve(x, k)
+    e = {m: oracle.rel_l2(oracle.fused_convolve_fp16_emulated(x, k, m), ref)
+         for m in ("pure", "storage", "mixed")}
+    assert e["storage"] < e["mixed"] and e["storage"] < e["pure"]
