@@ -94,11 +94,27 @@ template <bool BF, bool SPLIT>
 __device__ __forceinline__ void split8(const float (&x)[8], typename MT<BF>::v8& hi,
                                        typename MT<BF>::v8& lo) {
     using E = typename MT<BF>::e;
+    if constexpr (SPLIT && !BF) {
+        // FP16 hi/lo with the packed round-toward-zero convert (two values
+        // per instruction): hi keeps the top 11 bits of x, x - hi is exact in
+        // FP32 and lo keeps its top 11, about 22 bits in all
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        E h = (E)x[i];
-        hi[i] = h;
-        if constexpr (SPLIT) lo[i] = (E)(x[i] - (float)h);
+        for (int i = 0; i < 8; i += 2) {
+            const h2 h = __builtin_bit_cast(h2, __builtin_amdgcn_cvt_pkrtz(x[i], x[i + 1]));
+            const h2 l = __builtin_bit_cast(
+                h2, __builtin_amdgcn_cvt_pkrtz(x[i] - (float)h.x, x[i + 1] - (float)h.y));
+            hi[i] = h.x;
+            hi[i + 1] = h.y;
+            lo[i] = l.x;
+            lo[i + 1] = l.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            E h = (E)x[i];
+            hi[i] = h;
+            if constexpr (SPLIT) lo[i] = (E)(x[i] - (float)h);
+        }
     }
 }
 
@@ -165,6 +181,16 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         dma_transform(reinterpret_cast<float2*>(stage),
                       reinterpret_cast<const float2*>(in + tr * N), kCount, threadIdx.x, 256);
     for (; tr < batch; tr += gridDim.x) {  // uniform per workgroup
+        // FP16 split: lane coordinates from an opaque copy of the thread
+        // index, so the LDS and HBM index arithmetic below is recomputed per
+        // transform instead of hoisted out of the loop and held in registers
+        // across it (which spills that form); the others are faster with it
+        // hoisted
+        const int tid = SPLIT && !BF ? fresh((int)threadIdx.x) : (int)threadIdx.x;
+        const int lane = tid & 63;
+        const int w = tid >> 6;
+        const int c = lane & 15;
+        const int g = lane >> 4;
         typename IO::T* dst = out + tr * N;
         __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA (and earlier stores) done
         ds_barrier();                   // every wave's DMA landed; previous transform done
@@ -188,7 +214,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         if (tr + gridDim.x < batch)
             dma_transform(reinterpret_cast<float2*>(stage),
                           reinterpret_cast<const float2*>(in + (tr + gridDim.x) * N), kCount,
-                          threadIdx.x, 256);
+                          tid, 256);
         int e = 0;
         if (m > 0.f) (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
         const float sc = ldexpf(1.f, -e);

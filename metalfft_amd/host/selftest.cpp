@@ -135,6 +135,16 @@ int main() {
             }
             report(std::sqrt(num / den) < 1e-6 && y.size() == s.size() + 63, "delta_filter_identity",
                    std::sqrt(num / den));
+            // FP16 storage mode (FusedConvolverTests.swift:43-45 bound 1e-3)
+            mifft::FusedConvolver half(delta, 4096, mifft::FusedConvolver::Precision::float16Storage);
+            auto yh = half.apply(s);
+            num = den = 0;
+            for (size_t i = 0; i < s.size(); ++i) {
+                num += (yh[i] - s[i]) * (double)(yh[i] - s[i]);
+                den += s[i] * (double)s[i];
+            }
+            report(std::sqrt(num / den) < 1e-3 && std::sqrt(num / den) > 1e-7,
+                   "delta_filter_fp16_storage", std::sqrt(num / den));
             report(mifft::Convolver::recommendedFFTSize(1023).value_or(0) == 1024 &&
                        !mifft::Convolver::recommendedFFTSize(16384).has_value(),
                    "recommended_fft_size", 0);
