@@ -34,9 +34,9 @@ def parse():
     p.add_argument("--batch", type=int, default=4096)
     p.add_argument("--sets", type=int, default=4, help="rotating in/out buffer sets")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--launch", choices=("stream", "graph"), default="stream",
-                   help="timed region: K launches from a Python loop (default) or one hipGraph "
-                        "replay of the K launches; the other is reported beside it")
+    p.add_argument("--launch", choices=("graph", "stream"), default="graph",
+                   help="timed region: one hipGraph replay of the K launches (default) or K "
+                        "launches from a Python loop; the other is reported beside it")
     p.add_argument("--kernel", default=None, help="pin a kernel variant by name")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the MFMA / ragged / convolve / SAR measurements")
@@ -70,7 +70,7 @@ def time_graph(fn, steps, warmup, torch):
     """The same `steps` launches captured once into a hipGraph and replayed:
     the GPU runs them back to back with no host launch jitter between them.
     Warm-up launches run eagerly first (plans and occupancy queries settle
-    outside the capture), then one untimed replay; HIP events on the replay
+    outside the capture), then two untimed replays; HIP events on the replay
     stream bracket the timed replay."""
     for i in range(warmup):
         fn(i, None)
@@ -80,6 +80,7 @@ def time_graph(fn, steps, warmup, torch):
         capture = torch.cuda.current_stream().cuda_stream
         for i in range(steps):
             fn(i, capture)
+    graph.replay()
     graph.replay()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
@@ -194,7 +195,9 @@ def extra_mfma(m, torch, O, n=4096, batch=4096):
                     y = dst[idx].float().cpu().numpy().view(np.complex64)
                 nbytes = (16 if io == 0 else 8) * n * batch
                 out[key] = {"transforms_per_s": batch / (ms * 1e-3), "ms": ms,
-                            "gbs": nbytes / (ms * 1e-3) / 1e9, "rel_l2": O.rel_l2(y, ref),
+                            "gbs": nbytes / (ms * 1e-3) / 1e9,
+                            "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            "rel_l2": O.rel_l2(y, ref),
                             "max_abs": O.max_abs(y, ref), "kernel": plan.kernel_name}
             except Exception as e:  # report, keep the other lines
                 out[key] = {"error": repr(e)}
@@ -216,7 +219,7 @@ def extra_mfma(m, torch, O, n=4096, batch=4096):
     return out
 
 
-def extra_ragged(m, torch, count=10000):
+def extra_ragged(m, torch, count=10000, cpu=True):
     lengths = (2 ** np.random.default_rng(1).integers(6, 15, count)).astype(np.int64)
     rp = m.RaggedFFT(lengths)
     tot = int(lengths.sum())
@@ -233,12 +236,33 @@ def extra_ragged(m, torch, count=10000):
             rp.execute(x, y, m.FORWARD, mode)
             rp.execute(y, z, m.INVERSE, mode)
         ms = time_fn(step, 10, 2, torch)
+        gbs = 2 * 16 * tot / (ms * 1e-3) / 1e9
         out[name] = {"ms_fwd_inv": ms, "transforms_per_s": 2 * count / (ms * 1e-3),
-                     "gbs": 2 * 16 * tot / (ms * 1e-3) / 1e9, "launches": rp.launches(mode)}
+                     "gbs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS, "launches": rp.launches(mode)}
+    if cpu:
+        # the oracle's FP32 restatement over the same lengths, grouped by size
+        from oracle import oracle as O
+        O.build()
+        xh = x.cpu().numpy()
+        offs = np.concatenate([[0], np.cumsum(lengths)])
+        groups = {}
+        for i, n in enumerate(lengths):
+            groups.setdefault(int(n), []).append(xh[offs[i]:offs[i + 1]])
+        groups = {n: np.stack(v) for n, v in groups.items()}
+        threads = min(16, os.cpu_count() or 1)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            for n, v in groups.items():
+                O.cpu_fft(O.cpu_fft(v, threads=threads), inverse=True, threads=threads)
+            ts.append(time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": 2 * count / float(np.median(ts)), "unit": "transforms/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "the same 10,000 transforms forward + inverse, median of 3"}
     return out
 
 
-def extra_convolve(m, torch, count=512, S=65536, K=1023, N=4096):
+def extra_convolve(m, torch, count=512, S=65536, K=1023, N=4096, cpu=True):
     g = torch.Generator(device="cuda").manual_seed(0)
     x = torch.randn((count, S), generator=g, device="cuda")
     h = torch.randn(K, generator=g, device="cuda") / np.sqrt(K)
@@ -257,10 +281,29 @@ def extra_convolve(m, torch, count=512, S=65536, K=1023, N=4096):
         ms = time_fn(lambda: c16.apply_device(x, y, S, count), 10, 2, torch)
         fp16[name] = {"forward_ms": ms, "forward_gbs_alg": fbytes / (ms * 1e-3) / 1e9}
         c16.close()
-    return {"signals": count, "samples": S, "taps": K, "fft": N,
-            "forward_ms": fwd, "forward_gbs_alg": fbytes / (fwd * 1e-3) / 1e9,
-            "backward_ms": bwd, "backward_gbs_alg": bbytes / (bwd * 1e-3) / 1e9,
-            "signals_per_s_fwd_bwd": count / ((fwd + bwd) * 1e-3), "fp16_forward": fp16}
+    out = {"signals": count, "samples": S, "taps": K, "fft": N,
+           "forward_ms": fwd, "forward_gbs_alg": fbytes / (fwd * 1e-3) / 1e9,
+           "forward_roofline_frac": fbytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "backward_ms": bwd, "backward_gbs_alg": bbytes / (bwd * 1e-3) / 1e9,
+           "backward_roofline_frac": bbytes / (bwd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "signals_per_s_fwd_bwd": count / ((fwd + bwd) * 1e-3), "fp16_forward": fp16}
+    if cpu:
+        # the oracle's FP32 overlap-add restatement (Convolve.swift:36-76),
+        # one core, on 16 of the 512 signals
+        from oracle import oracle as O
+        O.build()
+        xs = x[:16].cpu().numpy()
+        hh = h.cpu().numpy()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            for r in range(16):
+                O.cpu_convolve(xs[r], hh, N)
+            ts.append(time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": 16 / float(np.median(ts)), "unit": "signals/s (forward)",
+                               "cores": 1, "kind": "port",
+                               "sample": "16 of the 512 signals, median of 3"}
+    return out
 
 
 def extra_sar(m, torch, n=4096):
@@ -271,7 +314,8 @@ def extra_sar(m, torch, n=4096):
     ms = time_fn(lambda: rda.process(raw, img), 10, 2, torch)
     nbytes = 3 * 2 * 4 * n * n  # three read+write sweeps of complex FP16
     return {"pulses": n, "range_samples": n, "io": "complex fp16", "ms": ms,
-            "frames_per_s": 1e3 / ms, "gbs_alg": nbytes / (ms * 1e-3) / 1e9}
+            "frames_per_s": 1e3 / ms, "gbs_alg": nbytes / (ms * 1e-3) / 1e9,
+            "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def load_traffic(name):
@@ -362,8 +406,9 @@ def main():
     extras = {}
     if not a.no_extras:
         for key, fn in (("mfma", lambda: extra_mfma(m, torch, O)),
-                        ("ragged_10k", lambda: extra_ragged(m, torch)),
-                        ("convolve_512x65536_k1023", lambda: extra_convolve(m, torch)),
+                        ("ragged_10k", lambda: extra_ragged(m, torch, cpu=not a.no_cpu)),
+                        ("convolve_512x65536_k1023",
+                         lambda: extra_convolve(m, torch, cpu=not a.no_cpu)),
                         ("sar_4096x4096_fp16", lambda: extra_sar(m, torch))):
             try:
                 extras[key] = fn()

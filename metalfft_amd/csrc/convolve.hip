@@ -83,28 +83,6 @@ __device__ __forceinline__ float ld_win(const Win& W, int n) {
     return __uint_as_float(__float_as_uint(v) & keep);
 }
 
-// The plan's twiddle memory holds packed rows for the radix-16 schedule
-// only (twiddle_table_len): E = 16 transforms take their table entries from
-// registers (PreBases, re-marked before each transform so derived powers
-// are not held from one to the next); E < 16 read the N-entry table.
-template <int N, int E>
-struct ConvTwiddles {
-    PreBases<N, E> pb;
-    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
-        if constexpr (E == 16) pb.load_packed(tw, t);
-    }
-    template <bool SYNC0>
-    __device__ __forceinline__ void fft(float2 (&v)[E], float2* buf, int t,
-                                        const float2* __restrict__ tw) {
-        if constexpr (E == 16) {
-            pb.opaque();
-            stockham_regs<N, E, LdsPad, 0, SYNC0, true, false, 2>(v, buf, t, tw, pb.w);
-        } else {
-            stockham_regs<N, E, LdsPad, 0, SYNC0>(v, buf, t, tw);
-        }
-    }
-};
-
 // --------------------------------------------------------------- forward
 template <int N, int E, int TPW>
 __global__ void __launch_bounds__((N / E) * TPW)
@@ -137,7 +115,7 @@ k_conv_fwd(const float* __restrict__ x, float* __restrict__ y, const float2* __r
         const float im = ld_win(W1, n);
         v[m] = make_float2(active ? re : 0.f, active && has1 ? im : 0.f);
     }
-    ConvTwiddles<N, E> ct;
+    RegTwiddles<N, E> ct;
     ct.load(tw, t);
     ct.template fft<false>(v, buf, t, tw);
     // multiply by the filter spectrum, then inverse via conj-FFT-conj
@@ -203,7 +181,7 @@ k_conv_bwd_x(const float* __restrict__ gy, float* __restrict__ gx, const float2*
     const bool has1 = 2 * q + 1 < nbx;
     float2 z[E];
     load_grad_pair<N, E>(z, gy + s * ylen, ylen, p0, p1, active, active && has1, t);
-    ConvTwiddles<N, E> ct;
+    RegTwiddles<N, E> ct;
     ct.load(tw, t);
     ct.template fft<false>(z, buf, t, tw);
 #pragma unroll
@@ -241,7 +219,7 @@ k_conv_bwd_h(const float* __restrict__ x, const float* __restrict__ gy,
     float2 acc[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) acc[m] = make_float2(0.f, 0.f);
-    ConvTwiddles<N, E> ct;
+    RegTwiddles<N, E> ct;
     ct.load(tw, t);
 
     for (int p = 0; p < ppw; ++p) {  // uniform trip count across the workgroup
@@ -287,14 +265,34 @@ k_conv_bwd_h(const float* __restrict__ x, const float* __restrict__ gy,
     }
 }
 
-// grad_h[k] = sum over slots w of part[w][k], in slot order.
-__global__ void k_sum_columns(const float* __restrict__ part, float* __restrict__ out, int64_t rows,
-                              int cols) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= cols) return;
+// Column sums of a rows x cols matrix, 64 rows per output row:
+// out[rb][c] = sum of in[r][c] over rb*64 <= r < min(rows, rb*64 + 64), as
+// four 16-row sums (independent loads in flight together) added in a fixed
+// order.  Applied until one row is left, it reduces the per-slot tap
+// gradient partials in the same order on every run (deterministic).
+__global__ void __launch_bounds__(256)
+k_sum_rows64(const float* __restrict__ in, float* __restrict__ out, int64_t rows, int cols) {
+    __shared__ float part[4][64];
+    const int cl = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cl;
+    const int cc = c < cols ? c : cols - 1;  // in-bounds column for the loads
+    const int64_t r0 = (int64_t)blockIdx.y * 64 + sub * 16;
+    float v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {  // unconditional (clamped) loads, masked sums
+        const int64_t r = r0 + i;
+        const float x = in[(r < rows ? r : rows - 1) * cols + cc];
+        uint32_t keep = r < rows ? 0xffffffffu : 0u;
+        __asm__("" : "+v"(keep));
+        v[i] = __uint_as_float(__float_as_uint(x) & keep);
+    }
     float s = 0.f;
-    for (int64_t r = 0; r < rows; ++r) s += part[r * cols + k];
-    out[k] = s;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += v[i];
+    part[sub][cl] = s;
+    __syncthreads();
+    if (sub == 0 && c < cols)
+        out[(int64_t)blockIdx.y * cols + c] = (part[0][cl] + part[1][cl]) + (part[2][cl] + part[3][cl]);
 }
 
 // --------------------------------------------------- FP16 precision modes
@@ -625,7 +623,7 @@ k_correlate(const float* __restrict__ a, int64_t na, const float* __restrict__ b
         const float im = nb > 0 ? ld_win(Bw, n) : 0.f;
         v[m] = make_float2(active ? re : 0.f, active ? im : 0.f);
     }
-    ConvTwiddles<N, E> ct;
+    RegTwiddles<N, E> ct;
     ct.load(tw, t);
     ct.template fft<false>(v, buf, t, tw);
     lds_barrier();
@@ -723,7 +721,9 @@ mifft_status convolver_backward(mifft_convolver* cv, const float* x, int64_t S, 
     if (ppw < 1) ppw = 1;
     const int64_t blocks = (items + ppw * tpw - 1) / (ppw * tpw);
     const int64_t slots = blocks * tpw;
-    mifft_status s = cv->partial.reserve(sizeof(float) * (size_t)slots * (size_t)cv->K);
+    // partials, then the 64-row sums of them (tree reduction scratch)
+    const int64_t level1 = (slots + 63) / 64;
+    mifft_status s = cv->partial.reserve(sizeof(float) * (size_t)(slots + 2 * level1) * (size_t)cv->K);
     if (s != MIFFT_OK) return s;
     float* part = static_cast<float*>(cv->partial.ptr);
     if ((s = check_hip(conv_bwd((int)cv->n, x, gy, gx, part, slots,
@@ -731,8 +731,23 @@ mifft_status convolver_backward(mifft_convolver* cv, const float* x, int64_t S, 
                                 nbx, npx, (int)ppw, st),
                        "fused convolve adjoint")) != MIFFT_OK)
         return s;
-    hipLaunchKernelGGL(k_sum_columns, dim3((unsigned)((cv->K + 255) / 256)), dim3(256), 0, st, part, gh,
-                       slots, (int)cv->K);
+    // grad_h = column sums of the slot partials: 64-row levels, ping-pong
+    // between two scratch areas after `part`, the last level into gh
+    const int K = (int)cv->K;
+    const float* src = part;
+    float* scratch[2] = {part + slots * (int64_t)K, part + (slots + level1) * (int64_t)K};
+    int64_t rows = slots;
+    for (int lvl = 0; rows > 1; ++lvl) {
+        const int64_t next = (rows + 63) / 64;
+        float* dst = next == 1 ? gh : scratch[lvl & 1];
+        hipLaunchKernelGGL(k_sum_rows64, dim3((unsigned)((K + 63) / 64), (unsigned)next), dim3(256), 0,
+                           st, src, dst, rows, K);
+        src = dst;
+        rows = next;
+    }
+    if (slots == 1)  // one slot: its partial is the gradient
+        hipLaunchKernelGGL(k_sum_rows64, dim3((unsigned)((K + 63) / 64), 1), dim3(256), 0, st, part, gh,
+                           (int64_t)1, K);
     return check_hip(hipGetLastError(), "tap gradient reduction");
 }
 

@@ -110,15 +110,21 @@ k_rows_compress(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out,
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = active ? SarIo<IO>::ld(src + t + T * m) : make_float2(0.f, 0.f);
-    stockham_regs<N, E, LdsPad, 0, false>(v, buf, t, tw);
+    RegTwiddles<N, E> rt;  // table entries in registers for both transforms
+    rt.load(tw, t);
+    rt.template fft<false>(v, buf, t, tw);
+    {
+        const int tf = fresh(t);  // filter index arithmetic not shared with the loads
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], filt.at(row, t + T * m)));
-    stockham_regs<N, E, LdsPad, 0, true>(v, buf, t, tw);
+        for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], filt.at(row, tf + T * m)));
+    }
+    rt.template fft<true>(v, buf, t, tw);
     if (!active) return;
     const float s = scale / (float)N;
     typename SarIo<IO>::T* dst = out + row * N;
+    const int ts = fresh(t);
 #pragma unroll
-    for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + t + T * m, make_float2(v[m].x * s, -v[m].y * s));
+    for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + ts + T * m, make_float2(v[m].x * s, -v[m].y * s));
 }
 
 // out[c][r] = in[r][c] for a rows x cols matrix, 64 x 64 tiles via LDS.

@@ -380,6 +380,28 @@ struct PreBases {
     }
 };
 
+// The plan's twiddle memory holds packed rows for the radix-16 schedule
+// only (twiddle_table_len): E = 16 transforms take their table entries from
+// registers (PreBases, re-marked before each transform so derived powers
+// are not held from one to the next); E < 16 read the N-entry table.
+template <int N, int E>
+struct RegTwiddles {
+    PreBases<N, E> pb;
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+        if constexpr (E == 16) pb.load_packed(tw, t);
+    }
+    template <bool SYNC0, class L = LdsPad>
+    __device__ __forceinline__ void fft(float2 (&v)[E], float2* buf, int t,
+                                        const float2* __restrict__ tw) {
+        if constexpr (E == 16) {
+            pb.opaque();
+            stockham_regs<N, E, L, 0, SYNC0, true, false, 2>(v, buf, t, tw, pb.w);
+        } else {
+            stockham_regs<N, E, L, 0, SYNC0>(v, buf, t, tw);
+        }
+    }
+};
+
 // One launch = ceil(batch / TPW) workgroups, TPW transforms each.  `in` and
 // `out` may alias (in-place): a transform's loads all precede its stores.
 // TWM, twiddles of passes >= 1: 0 = table reads at each pass (TW4 or all
