@@ -303,6 +303,14 @@ MIFFT_API mifft_status mifft_rda_corner_turn_device(const mifft_rda *rda, const 
 MIFFT_API mifft_status mifft_rda_azimuth_compress_device(const mifft_rda *rda, const void *in,
                                                          void *out, void *stream);
 
+/* ------------------------------------------------------------ diagnostic */
+/* No counterpart in the reference.  Copies `bytes` from src to dst (device
+ * memory, not overlapping) with the access style of the FFT kernels
+ * (16-byte non-temporal loads and stores, one-shot workgroups); the rate it
+ * reaches is the achievable HBM streaming ceiling that bench.py reads the
+ * transforms' roofline fractions against.  Asynchronous on `stream`. */
+MIFFT_API mifft_status mifft_copy_device(const void *src, void *dst, int64_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

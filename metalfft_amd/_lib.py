@@ -110,6 +110,7 @@ _SIGNATURES = [
     ("mifft_correlator_auto_host", c_int, [c_void_p, fptr, c_int64, fptr]),
     ("mifft_correlator_cross_device", c_int,
      [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
+    ("mifft_copy_device", c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
 ]
 
 _lib = None

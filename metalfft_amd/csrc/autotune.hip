@@ -6,7 +6,7 @@
 // for batched N=4096 in Sources/MetalFFT/FFT/MetalFFT.swift:163-165); this
 // replaces it with measurement.
 //
-// Cache: text file, first line "mifft-autotune 1 <arch>", then one
+// Cache: text file, first line "mifft-autotune 2 <arch>", then one
 // "<kind> <dtype> <n> <variant-name> <ns-per-transform>" line per entry.
 // Location: $MIFFT_AUTOTUNE_CACHE, else $XDG_CACHE_HOME/mifft/autotune.txt,
 // else $HOME/.cache/mifft/autotune.txt.  A header mismatch (other version or
@@ -30,7 +30,7 @@
 namespace mifft {
 
 namespace {
-const char* kHeader = "mifft-autotune 1 gfx950";
+const char* kHeader = "mifft-autotune 2 gfx950";
 
 std::mutex g_tune_mutex;
 bool g_loaded = false;
@@ -112,8 +112,23 @@ const StockhamVariant* by_name(const std::string& name) {
 }
 
 // Times each candidate on ~32 MiB of data; returns the fastest.
+// Timing input: uniform values in [-1, 1) from an integer hash (data with
+// real bit activity, as transforms see; zeros would flatter every kernel).
+__global__ void k_fill_noise(float2* __restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    a[i] = make_float2((float)(h & 0xffff) / 32768.f - 1.f, (float)(h >> 16) / 32768.f - 1.f);
+}
+
+// Timed on 2^25 complex values (256 MiB in, 256 MiB out): past the 256 MiB
+// Infinity Cache, so the choice is made at HBM speed, where batches of this
+// path run.
 const StockhamVariant* time_variants(int n, hipStream_t stream, const float2* tw, double* best_ns) {
-    const int64_t batch = std::max<int64_t>(1, (int64_t(1) << 22) / n);
+    const int64_t batch = std::max<int64_t>(1, (int64_t(1) << 25) / n);
     float2 *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, sizeof(float2) * n * batch) != hipSuccess ||
         hipMalloc(&b, sizeof(float2) * n * batch) != hipSuccess) {
@@ -121,7 +136,9 @@ const StockhamVariant* time_variants(int n, hipStream_t stream, const float2* tw
         if (a) (void)hipFree(a);
         return nullptr;
     }
-    (void)hipMemsetAsync(a, 0, sizeof(float2) * n * batch, stream);
+    const int64_t total = (int64_t)n * batch;
+    hipLaunchKernelGGL(k_fill_noise, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, a,
+                       total);
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);

@@ -15,7 +15,9 @@
 //   _swz_nt_pt    : _swz_nt whose threads load their twiddle table entries
 //                   for every pass right after the transform's data, as
 //                   16-byte reads of packed per-pass rows (PassTable), so no
-//                   pass waits on an L2 round trip for them.
+//                   pass waits on an L2 round trip for them;
+//   _stream_pt    : _stream with those entries held in registers for the
+//                   whole loop.
 // All read twiddles as W^k, W^2k, W^4k, W^8k and derive the other powers.
 // The autotuner picks one per size (autotune.hip).
 #include "kernels.hpp"
@@ -77,7 +79,9 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
         static int per_cu = [] {
             int nb = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &nb, reinterpret_cast<const void*>(&k_stockham_stream<N, E, TPW, false, L, NT>),
+                    &nb,
+                    reinterpret_cast<const void*>(
+                        &k_stockham_stream<N, E, TPW, false, L, NT, true, 0, TWM>),
                     threads, 0) != hipSuccess || nb <= 0)
                 nb = 1;
             return nb;
@@ -85,11 +89,11 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
         const int64_t resident = (int64_t)per_cu * device_cu_count();
         if (blocks > resident) blocks = resident;
         if (inverse)
-            hipLaunchKernelGGL((k_stockham_stream<N, E, TPW, true, L, NT>), dim3((unsigned)blocks),
-                               dim3(threads), 0, stream, in, out, tw, batch);
+            hipLaunchKernelGGL((k_stockham_stream<N, E, TPW, true, L, NT, true, 0, TWM>),
+                               dim3((unsigned)blocks), dim3(threads), 0, stream, in, out, tw, batch);
         else
-            hipLaunchKernelGGL((k_stockham_stream<N, E, TPW, false, L, NT>), dim3((unsigned)blocks),
-                               dim3(threads), 0, stream, in, out, tw, batch);
+            hipLaunchKernelGGL((k_stockham_stream<N, E, TPW, false, L, NT, true, 0, TWM>),
+                               dim3((unsigned)blocks), dim3(threads), 0, stream, in, out, tw, batch);
     } else {
         if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
         if (inverse)
@@ -115,6 +119,10 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
 #define MIFFT_VT(N)                                                                             \
     {N, 16, tpw_for<N, 16>(), &launch_stockham<N, 16, LdsSwz, true, false, 2>,                  \
      "stockham_r16_" #N "_swz_nt_pt"}
+// _stream with the packed rows in registers for the whole loop
+#define MIFFT_VPT(N)                                                                            \
+    {N, 16, tpw_for<N, 16>(), &launch_stockham<N, 16, LdsSwz, true, true, 2>,                   \
+     "stockham_r16_" #N "_stream_pt"}
 #define MIFFT_VD(N, E) \
     {N, E, tpw_for<N, E>(), &launch_dma<N, E>, "stockham_r" #E "_" #N "_dma"}
 
@@ -138,12 +146,14 @@ static const StockhamVariant kVariants[] = {
     MIFFT_V(16384, 16), MIFFT_VS(16384, 16), MIFFT_VP(16384, 16),
     MIFFT_VT(64),      MIFFT_VT(128),     MIFFT_VT(256),      MIFFT_VT(512),      MIFFT_VT(1024),
     MIFFT_VT(2048),    MIFFT_VT(4096),    MIFFT_VT(8192),     MIFFT_VT(16384),
+    MIFFT_VPT(1024),   MIFFT_VPT(2048),   MIFFT_VPT(4096),    MIFFT_VPT(8192),
 };
 #undef MIFFT_V
 #undef MIFFT_VS
 #undef MIFFT_VP
 #undef MIFFT_VD
 #undef MIFFT_VT
+#undef MIFFT_VPT
 
 const StockhamVariant* stockham_variants(int* count) {
     *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));

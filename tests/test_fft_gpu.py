@@ -203,3 +203,22 @@ def test_twiddle_generation_matches_fp64(gpu, oracle):
         y = gpu.MetalFFT(n).forward(x)
         tw = oracle.twiddle_table(n)
         assert np.max(np.abs(y - tw)) < 4e-7
+
+
+def test_diagnostic_copy(gpu):
+    """mifft_copy_device (the bench's HBM copy ceiling) copies exactly, full
+    blocks, a partial last block and a byte tail, and rejects bad sizes."""
+    import ctypes
+    import torch
+    from metalfft_amd import _lib
+    L = _lib.load()
+    for nbytes in (16, 32768 * 3, 32768 * 3 + 4096 + 48, 1 << 24, (1 << 20) + 7):
+        src = torch.randint(0, 255, (nbytes,), dtype=torch.uint8, device="cuda")
+        dst = torch.zeros_like(src)
+        rc = L.mifft_copy_device(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                 ctypes.c_int64(nbytes), None)
+        assert rc == 0
+        torch.cuda.synchronize()
+        assert torch.equal(src, dst), nbytes
+    assert L.mifft_copy_device(None, None, ctypes.c_int64(-1), None) != 0
+    assert L.mifft_copy_device(None, None, ctypes.c_int64(0), None) == 0
