@@ -328,15 +328,21 @@ def extra_convolve(m, torch, count=512, S=65536, K=1023, N=4096, cpu=True):
 
 
 def extra_sar(m, torch, n=4096):
-    rda = m.RangeDoppler(m.RdaParams.defaults(n_range=n, n_azimuth=n, io=1))
+    """Config 5 (range compression -> corner turn -> azimuth compression, no
+    RCMC) and the full RDA with RCMC, complex FP16, 4096 x 4096."""
     g = torch.Generator(device="cuda").manual_seed(5)
     raw = (torch.randn((n, 2 * n), generator=g, device="cuda") * 0.1).half()
     img = torch.empty_like(raw)
-    ms = time_fn(lambda: rda.process(raw, img), 10, 2, torch)
-    nbytes = 3 * 2 * 4 * n * n  # three read+write sweeps of complex FP16
-    return {"pulses": n, "range_samples": n, "io": "complex fp16", "ms": ms,
-            "frames_per_s": 1e3 / ms, "gbs_alg": nbytes / (ms * 1e-3) / 1e9,
-            "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out = {"pulses": n, "range_samples": n, "io": "complex fp16"}
+    for key, rcmc, sweeps in (("config5_no_rcmc", 0, 3), ("full_rda_rcmc", 1, 5)):
+        rda = m.RangeDoppler(m.RdaParams.defaults(n_range=n, n_azimuth=n, io=1, rcmc=rcmc))
+        ms = time_fn(lambda: rda.process(raw, img), 10, 2, torch)
+        nbytes = sweeps * 2 * 4 * n * n  # read + write sweeps of complex FP16
+        out[key] = {"ms": ms, "frames_per_s": 1e3 / ms, "sweeps": sweeps,
+                    "gbs_alg": nbytes / (ms * 1e-3) / 1e9,
+                    "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        rda.close()
+    return out
 
 
 def load_traffic(name):

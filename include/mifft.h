@@ -269,19 +269,25 @@ typedef struct mifft_rda_params {
     double center_range;   /* scene centre slant range, m */
     int io;                /* mifft_io: MIFFT_IO_C32H (complex FP16) or MIFFT_IO_C64 */
     int filter_convention; /* MIFFT_RDA_MATCHED or MIFFT_RDA_REFERENCE_FORMULA */
+    int rcmc;              /* 1: range cell migration correction between the azimuth FFT and
+                              the azimuth filter (rda_kernels.metal:103-185), 0: none */
 } mifft_rda_params;
 
-/* Azimuth filter phase sign.  MATCHED: H_a = exp(+i phi), the conjugate of the
- * echo's Doppler phase exp(-i phi) (echo phase -4 pi R / lambda,
- * sar_simulator.swift:144), which focuses the targets.  REFERENCE_FORMULA:
- * H_a = exp(-i phi), the literal formula of rda_kernels.metal:231-232, kept
- * for comparison with the reference (it does not compress in azimuth). */
+/* Azimuth filter phase sign and RCMC direction.  MATCHED: H_a = exp(+i phi),
+ * the conjugate of the echo's Doppler phase exp(-i phi) (echo phase
+ * -4 pi R / lambda, sar_simulator.swift:144), which focuses the targets, and
+ * RCMC reading range bin r + shift (migrated energy pulled back).
+ * REFERENCE_FORMULA: the literal formulas of rda_kernels.metal:231-232
+ * (H_a = exp(-i phi)) and :150 (srcIdx = rgIdx - shift), kept for comparison
+ * with the reference (they do not focus: the filter does not compress in
+ * azimuth and the RCMC doubles the migration). */
 #define MIFFT_RDA_MATCHED 0
 #define MIFFT_RDA_REFERENCE_FORMULA 1
 
 /* The reference's defaults (sar_simulator.swift:22-67): 4096 x 4096,
  * B = 100 MHz, Tp = 10 us, fs = 1.2 B, PRF = 1 kHz, v = 100 m/s, fc = 10 GHz,
- * R0 = 20 km; complex-FP16 data. */
+ * R0 = 20 km; complex-FP16 data; RCMC on (the reference's RDA pipelines
+ * always run it, rda_fused_pipeline.swift:158-163). */
 MIFFT_API mifft_status mifft_rda_default_params(mifft_rda_params *p);
 MIFFT_API mifft_status mifft_rda_create(const mifft_rda_params *p, mifft_rda **rda);
 MIFFT_API mifft_status mifft_rda_destroy(mifft_rda *rda);
@@ -295,7 +301,10 @@ MIFFT_API mifft_status mifft_rda_scales(const mifft_rda *rda, double *range_scal
  * Asynchronous on `stream`; uses a plan-owned work buffer of one frame. */
 MIFFT_API mifft_status mifft_rda_process_device(mifft_rda *rda, const void *raw, void *image,
                                                 void *stream);
-/* The three steps separately (same layouts; the corner turn is out of place). */
+/* The three steps separately (same layouts; the corner turn is out of place).
+ * With rcmc, azimuth compression is azimuth FFT -> RCMC -> filter -> IFFT,
+ * through the plan's work frame (in and out may be the same buffer); calls
+ * on one plan must not overlap. */
 MIFFT_API mifft_status mifft_rda_range_compress_device(const mifft_rda *rda, const void *raw,
                                                        void *out, void *stream);
 MIFFT_API mifft_status mifft_rda_corner_turn_device(const mifft_rda *rda, const void *in, void *out,

@@ -92,11 +92,50 @@ constexpr int rows_e() { return N >= 256 ? 16 : 8; }
 template <int N>
 constexpr int rows_threads() { return (N / rows_e<N>()) * tpw_for<N, rows_e<N>()>(); }
 
-// One workgroup per row (TPW rows for small N): FFT -> x filter -> IFFT.
-template <int N, int IO, class F>
+// RCMC on the range-Doppler frame laid out rows = range bins, columns =
+// Doppler bins (the layout after the corner turn and azimuth FFT):
+//   out[r][f] = sum_{k=0..7} w[f][k] in[r + si[f] + k - 3][f],
+// taps outside the frame dropped (rda_kernels.metal:150-180).  The shift is
+// the same for every range bin, so si and the 8 windowed-sinc weights per
+// Doppler bin are formed once on the host in FP64.  A workgroup is 64
+// Doppler columns x 4 range bins: every tap load is one coalesced row
+// segment, and the workgroups of a column tile (same XCD, since ids
+// x + 64 y share x mod 8) reuse each other's rows from L2.
+template <int IO>
+__global__ void __launch_bounds__(256)
+k_rcmc(const typename SarIo<IO>::T* __restrict__ in, typename SarIo<IO>::T* __restrict__ out,
+       const int* __restrict__ si, const float4* __restrict__ w, int64_t nr, int64_t na) {
+    const int64_t f = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int64_t r = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (f >= na || r >= nr) return;
+    const int s = si[f];
+    const float4 wa = w[2 * f], wb = w[2 * f + 1];
+    const float wk[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t rr = r + s + k - 3;
+        const bool ok = rr >= 0 && rr < nr;
+        const float2 x = SarIo<IO>::ld(in + (rr < 0 ? 0 : (rr >= nr ? nr - 1 : rr)) * na + f);
+        uint32_t m = ok ? 0xffffffffu : 0u;  // unconditional load, opaque mask
+        __asm__("" : "+v"(m));
+        acc.x += wk[k] * __uint_as_float(__float_as_uint(x.x) & m);
+        acc.y += wk[k] * __uint_as_float(__float_as_uint(x.y) & m);
+    }
+    SarIo<IO>::st(out + r * na + f, acc);
+}
+
+enum { kRowsCompress = 0, kRowsForward = 1, kRowsFilterInverse = 2 };
+
+// One workgroup per row (TPW rows for small N):
+//   kRowsCompress       FFT -> x filter -> IFFT, times scale;
+//   kRowsForward        FFT, times scale (the azimuth FFT before RCMC);
+//   kRowsFilterInverse  x filter -> IFFT, times scale (after RCMC).
+// A row is read whole before it is written: in == out is allowed.
+template <int N, int IO, class F, int MODE>
 __global__ void __launch_bounds__(rows_threads<N>())
-k_rows_compress(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out,
-                const float2* __restrict__ tw, F filt, int64_t rows, float scale) {
+k_rows(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2* __restrict__ tw,
+       F filt, int64_t rows, float scale) {
     constexpr int E = N >= 256 ? 16 : 8;
     constexpr int TPW = tpw_for<N, E>();
     constexpr int T = N / E;
@@ -106,25 +145,37 @@ k_rows_compress(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out,
     const int64_t row = (int64_t)blockIdx.x * TPW + local;
     const bool active = row < rows;
     float2* buf = lds + local * LdsPad::len(N);
-    const typename SarIo<IO>::T* src = in + (active ? row : 0) * N;
     float2 v[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = active ? SarIo<IO>::ld(src + t + T * m) : make_float2(0.f, 0.f);
     RegTwiddles<N, E> rt;  // table entries in registers for both transforms
     rt.load(tw, t);
-    rt.template fft<false>(v, buf, t, tw);
     {
-        const int tf = fresh(t);  // filter index arithmetic not shared with the loads
+        const typename SarIo<IO>::T* src = in + (active ? row : 0) * N;
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], filt.at(row, tf + T * m)));
+        for (int m = 0; m < E; ++m)
+            v[m] = active ? SarIo<IO>::ld(src + t + T * m) : make_float2(0.f, 0.f);
+        if constexpr (MODE != kRowsFilterInverse) rt.template fft<false>(v, buf, t, tw);
     }
-    rt.template fft<true>(v, buf, t, tw);
-    if (!active) return;
-    const float s = scale / (float)N;
-    typename SarIo<IO>::T* dst = out + row * N;
-    const int ts = fresh(t);
+    if constexpr (MODE == kRowsForward) {
+        if (!active) return;
+        typename SarIo<IO>::T* dst = out + row * N;
+        const int ts = fresh(t);
 #pragma unroll
-    for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + ts + T * m, make_float2(v[m].x * s, -v[m].y * s));
+        for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + ts + T * m, make_float2(v[m].x * scale, v[m].y * scale));
+        return;
+    } else {
+        {
+            const int tf = fresh(t);  // filter index arithmetic not shared with the loads
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], filt.at(row, tf + T * m)));
+        }
+        rt.template fft<MODE == kRowsCompress>(v, buf, t, tw);  // SYNC0: LDS used by the forward
+        if (!active) return;
+        const float s = scale / (float)N;
+        typename SarIo<IO>::T* dst = out + row * N;
+        const int ts = fresh(t);
+#pragma unroll
+        for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + ts + T * m, make_float2(v[m].x * s, -v[m].y * s));
+    }
 }
 
 // out[c][r] = in[r][c] for a rows x cols matrix, 64 x 64 tiles via LDS.
@@ -149,41 +200,54 @@ k_corner_turn(const typename SarIo<IO>::T* __restrict__ in, typename SarIo<IO>::
     }
 }
 
-template <int N, int IO, class F>
+template <int N, int IO, class F, int MODE>
 hipError_t rows_n(const void* in, void* out, const float2* tw, F f, int64_t rows, float scale,
                   hipStream_t st) {
     constexpr int E = N >= 256 ? 16 : 8;
     constexpr int TPW = tpw_for<N, E>();
     const int64_t blocks = (rows + TPW - 1) / TPW;
     if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_rows_compress<N, IO, F>), dim3((unsigned)blocks), dim3((N / E) * TPW), 0, st,
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_rows<N, IO, F, MODE>), dim3((unsigned)blocks), dim3((N / E) * TPW), 0, st,
                        static_cast<const typename SarIo<IO>::T*>(in),
                        static_cast<typename SarIo<IO>::T*>(out), tw, f, rows, scale);
     return hipGetLastError();
 }
 
-template <int IO, class F>
+template <int IO, class F, int MODE>
 hipError_t rows_dispatch(int n, const void* in, void* out, const float2* tw, F f, int64_t rows,
                          float scale, hipStream_t st) {
     switch (n) {
-        case 64: return rows_n<64, IO, F>(in, out, tw, f, rows, scale, st);
-        case 128: return rows_n<128, IO, F>(in, out, tw, f, rows, scale, st);
-        case 256: return rows_n<256, IO, F>(in, out, tw, f, rows, scale, st);
-        case 512: return rows_n<512, IO, F>(in, out, tw, f, rows, scale, st);
-        case 1024: return rows_n<1024, IO, F>(in, out, tw, f, rows, scale, st);
-        case 2048: return rows_n<2048, IO, F>(in, out, tw, f, rows, scale, st);
-        case 4096: return rows_n<4096, IO, F>(in, out, tw, f, rows, scale, st);
-        case 8192: return rows_n<8192, IO, F>(in, out, tw, f, rows, scale, st);
-        case 16384: return rows_n<16384, IO, F>(in, out, tw, f, rows, scale, st);
+        case 64: return rows_n<64, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 128: return rows_n<128, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 256: return rows_n<256, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 512: return rows_n<512, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 1024: return rows_n<1024, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 2048: return rows_n<2048, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 4096: return rows_n<4096, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 8192: return rows_n<8192, IO, F, MODE>(in, out, tw, f, rows, scale, st);
+        case 16384: return rows_n<16384, IO, F, MODE>(in, out, tw, f, rows, scale, st);
     }
     return hipErrorInvalidValue;
 }
 
-template <class F>
+template <class F, int MODE = kRowsCompress>
 hipError_t rows_io(int io, int n, const void* in, void* out, const float2* tw, F f, int64_t rows,
                    float scale, hipStream_t st) {
-    return io == MIFFT_IO_C32H ? rows_dispatch<MIFFT_IO_C32H>(n, in, out, tw, f, rows, scale, st)
-                               : rows_dispatch<MIFFT_IO_C64>(n, in, out, tw, f, rows, scale, st);
+    return io == MIFFT_IO_C32H ? rows_dispatch<MIFFT_IO_C32H, F, MODE>(n, in, out, tw, f, rows, scale, st)
+                               : rows_dispatch<MIFFT_IO_C64, F, MODE>(n, in, out, tw, f, rows, scale, st);
+}
+
+hipError_t rcmc(int io, const void* in, void* out, const int* si, const float4* w, int64_t nr,
+                int64_t na, hipStream_t st) {
+    dim3 grid((unsigned)((na + 63) / 64), (unsigned)((nr + 3) / 4));
+    if (io == MIFFT_IO_C32H)
+        hipLaunchKernelGGL(k_rcmc<MIFFT_IO_C32H>, grid, dim3(256), 0, st, static_cast<const h2*>(in),
+                           static_cast<h2*>(out), si, w, nr, na);
+    else
+        hipLaunchKernelGGL(k_rcmc<MIFFT_IO_C64>, grid, dim3(256), 0, st, static_cast<const float2*>(in),
+                           static_cast<float2*>(out), si, w, nr, na);
+    return hipGetLastError();
 }
 
 hipError_t corner_turn(int io, const void* in, void* out, int64_t rows, int64_t cols, hipStream_t st) {
@@ -220,9 +284,27 @@ mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStrea
     AzimuthFilter f{static_cast<const double*>(r->az_a.ptr), static_cast<const double*>(r->az_d.ptr),
                     r->az_b, r->p.n_range / 2,
                     r->p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0f : 1.0f};
-    return check_hip(rows_io(r->p.io, (int)r->p.n_azimuth, in, out, r->az_plan->twiddles, f,
-                             r->p.n_range, (float)r->az_scale, st),
-                     "azimuth compression");
+    if (!r->p.rcmc)
+        return check_hip(rows_io(r->p.io, (int)r->p.n_azimuth, in, out, r->az_plan->twiddles, f,
+                                 r->p.n_range, (float)r->az_scale, st),
+                         "azimuth compression");
+    // azimuth FFT (scaled: the range-Doppler frame is stored, FP16 must not
+    // overflow) into the work frame, RCMC from it into out, then filter +
+    // IFFT in place
+    mifft_status s = check_hip(rows_io<AzimuthFilter, kRowsForward>(
+                                   r->p.io, (int)r->p.n_azimuth, in, r->work.ptr,
+                                   r->az_plan->twiddles, f, r->p.n_range, (float)r->az_scale, st),
+                               "azimuth FFT");
+    if (s != MIFFT_OK) return s;
+    if ((s = check_hip(rcmc(r->p.io, r->work.ptr, out, static_cast<const int*>(r->rcmc_si.ptr),
+                            static_cast<const float4*>(r->rcmc_w.ptr), r->p.n_range, r->p.n_azimuth,
+                            st),
+                       "RCMC")) != MIFFT_OK)
+        return s;
+    return check_hip(rows_io<AzimuthFilter, kRowsFilterInverse>(r->p.io, (int)r->p.n_azimuth, out, out,
+                                                                 r->az_plan->twiddles, f, r->p.n_range,
+                                                                 1.0f, st),
+                     "azimuth filter + IFFT");
 }
 
 // Chirp reference of src/radar/sar_simulator.swift:207-224: Tp*fs samples of
@@ -298,6 +380,45 @@ mifft_status rda_setup(mifft_rda* r) {
         (s = check_hip(hipMemcpy(r->az_d.ptr, D.data(), ab, hipMemcpyHostToDevice), "hipMemcpy")) !=
             MIFFT_OK)
         return s;
+    // RCMC (rda_kernels.metal:103-185): shift per Doppler bin in range bins,
+    // dR = R0 (1/sqrt(1 - ratio^2) - 1) at the centre range, 0 where
+    // |ratio| >= 0.95; output bin r reads r + shift (matched: migrated energy
+    // pulled back) or r - shift (the kernel's literal srcIdx = rgIdx - shift).
+    // Integer part and the 8 windowed-sinc weights of the fraction, in FP64.
+    if (p.rcmc) {
+        std::vector<int> si((size_t)p.n_azimuth);
+        std::vector<float> w((size_t)p.n_azimuth * 8);
+        const double sgn = p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0 : 1.0;
+        for (int64_t k = 0; k < p.n_azimuth; ++k) {
+            const double fa_norm = k < p.n_azimuth / 2 ? (double)k / (double)p.n_azimuth
+                                                       : (double)(k - p.n_azimuth) / (double)p.n_azimuth;
+            const double ratio = lambda * fa_norm * p.prf / (2.0 * p.velocity);
+            const double dr =
+                std::fabs(ratio) < 0.95 ? p.center_range * (1.0 / std::sqrt(1.0 - ratio * ratio) - 1.0) : 0.0;
+            const double shift = sgn * dr / range_pixel;
+            const double fl = std::floor(shift);
+            const double frac = shift - fl;
+            si[(size_t)k] = (int)fl;
+            for (int j = 0; j < 8; ++j) {
+                const double x = frac - (double)(j - 3);
+                const double sinc = std::fabs(x) < 1e-6 ? 1.0 : std::sin(M_PI * x) / (M_PI * x);
+                const double win = 0.54 - 0.46 * std::cos(2.0 * M_PI * (double)j / 7.0);
+                w[(size_t)k * 8 + j] = (float)(sinc * win);
+            }
+        }
+        if ((s = r->rcmc_si.reserve(sizeof(int) * si.size())) != MIFFT_OK ||
+            (s = r->rcmc_w.reserve(sizeof(float) * w.size())) != MIFFT_OK)
+            return s;
+        if ((s = check_hip(hipMemcpy(r->rcmc_si.ptr, si.data(), sizeof(int) * si.size(),
+                                     hipMemcpyHostToDevice), "hipMemcpy")) != MIFFT_OK ||
+            (s = check_hip(hipMemcpy(r->rcmc_w.ptr, w.data(), sizeof(float) * w.size(),
+                                     hipMemcpyHostToDevice), "hipMemcpy")) != MIFFT_OK)
+            return s;
+    }
+    // one frame of work memory: range-compressed data in process_device, the
+    // range-Doppler frame before RCMC
+    const size_t esz = p.io == MIFFT_IO_C32H ? 4 : 8;
+    if ((s = r->work.reserve(esz * (size_t)p.n_range * (size_t)p.n_azimuth)) != MIFFT_OK) return s;
     // power-of-two stage gains keep complex FP16 in range: a point target's
     // range peak is ~(chirp samples), its azimuth peak ~(pulses) times that
     r->chirp_samples = ns;
@@ -326,6 +447,7 @@ mifft_status mifft_rda_default_params(mifft_rda_params* p) {
     p->center_range = 20000.0;
     p->io = MIFFT_IO_C32H;
     p->filter_convention = MIFFT_RDA_MATCHED;
+    p->rcmc = 1;
     return MIFFT_OK;
 }
 
@@ -343,6 +465,7 @@ mifft_status mifft_rda_create(const mifft_rda_params* params, mifft_rda** out) {
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown io type");
     if (p.filter_convention != MIFFT_RDA_MATCHED && p.filter_convention != MIFFT_RDA_REFERENCE_FORMULA)
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown filter convention");
+    if (p.rcmc != 0 && p.rcmc != 1) return fail(MIFFT_ERR_INVALID_ARGUMENT, "rcmc must be 0 or 1");
     if (!(p.bandwidth > 0 && p.pulse_duration > 0 && p.sampling_rate > 0 && p.prf > 0 &&
           p.velocity > 0 && p.carrier_freq > 0 && p.center_range > 0))
         return fail(MIFFT_ERR_PRECONDITION, "SAR parameters must be positive");
@@ -372,6 +495,8 @@ mifft_status mifft_rda_destroy(mifft_rda* r) {
     r->range_filter.release();
     r->az_a.release();
     r->az_d.release();
+    r->rcmc_si.release();
+    r->rcmc_w.release();
     r->work.release();
     mifft_plan_destroy(r->range_plan);
     mifft_plan_destroy(r->az_plan);
@@ -411,9 +536,7 @@ mifft_status mifft_rda_azimuth_compress_device(const mifft_rda* r, const void* i
 mifft_status mifft_rda_process_device(mifft_rda* r, const void* raw, void* image, void* stream) {
     if (!r || !raw || !image) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
     if (raw == image) return fail(MIFFT_ERR_INVALID_ARGUMENT, "raw and image must differ");
-    const size_t esz = r->p.io == MIFFT_IO_C32H ? 4 : 8;
-    mifft_status s = r->work.reserve(esz * (size_t)r->p.n_range * (size_t)r->p.n_azimuth);
-    if (s != MIFFT_OK) return s;
+    mifft_status s;
     hipStream_t st = static_cast<hipStream_t>(stream);
     if ((s = rda_range(r, raw, r->work.ptr, st)) != MIFFT_OK) return s;
     if ((s = rda_turn(r, r->work.ptr, image, st)) != MIFFT_OK) return s;

@@ -11,6 +11,8 @@ struct mifft_rda {
     double az_b = 0.0;                 // cycles per range bin per unit D
     double range_scale = 1.0, az_scale = 1.0;
     int64_t chirp_samples = 0;
-    mifft::DeviceBuffer work;          // range-compressed data (process_device)
+    mifft::DeviceBuffer rcmc_si, rcmc_w;  // RCMC: integer shift and 8 weights per Doppler bin
+    mifft::DeviceBuffer work;          // one frame: range-compressed data (process_device), the
+                                       // range-Doppler frame before RCMC
     hipStream_t stream = nullptr;
 };

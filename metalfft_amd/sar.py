@@ -1,8 +1,9 @@
 """Python mirror of the SAR range-Doppler processor over the libmifft C-ABI.
 
 Counterpart of the reference's RDA pipelines (src/radar/rda_fused_pipeline.swift,
-src/radar/rda_pipeline.swift) without RCMC: range compression, corner turn,
-azimuth compression, on complex-FP16 (default) or complex64 device data.
+src/radar/rda_pipeline.swift): range compression, corner turn, azimuth FFT,
+range cell migration correction (``rcmc=1``, default), azimuth filter and
+IFFT, on complex-FP16 (default) or complex64 device data.
 """
 from __future__ import annotations
 
@@ -23,7 +24,7 @@ class RdaParams(ctypes.Structure):
                 ("sampling_rate", ctypes.c_double), ("prf", ctypes.c_double),
                 ("velocity", ctypes.c_double), ("carrier_freq", ctypes.c_double),
                 ("center_range", ctypes.c_double), ("io", ctypes.c_int),
-                ("filter_convention", ctypes.c_int)]
+                ("filter_convention", ctypes.c_int), ("rcmc", ctypes.c_int)]
 
     @classmethod
     def defaults(cls, **overrides) -> "RdaParams":

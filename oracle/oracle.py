@@ -343,16 +343,79 @@ def sar_azimuth_filter(p, reference_formula=False):
     return h
 
 
-def f64_rda(raw, p, range_scale=1.0, azimuth_scale=1.0, reference_formula=False):
-    """Range compression -> corner turn -> azimuth compression in FP64;
-    image is range-bin-major (Nr x Na)."""
+def sar_rcmc_shift(p, reference_formula=False):
+    """Signed source offset, in range bins, per azimuth-frequency bin for the
+    RCMC of src/radar/rda_kernels.metal:103-185, in FP64: the migration
+    dR(f_a) = R0 (1 / sqrt(1 - ratio^2) - 1) (R0 the centre range, ratio =
+    lambda f_a / 2V with f_a = fa_norm * PRF in FFT order, dR = 0 where
+    |ratio| >= 0.95) over the range pixel c / 2 fs.  The output bin r reads
+    r + shift: energy migrated to a longer range is pulled back (+dR); with
+    reference_formula the kernel's literal srcIdx = rgIdx - shift (-dR)."""
+    na = p["n_azimuth"]
+    lam = C_LIGHT / p["carrier_freq"]
+    pixel = C_LIGHT / (2.0 * p["sampling_rate"])
+    k = np.arange(na)
+    fa = np.where(k < na // 2, k, k - na) / na * p["prf"]
+    ratio = lam * fa / (2.0 * p["velocity"])
+    dr = np.where(np.abs(ratio) < 0.95,
+                  p["center_range"] * (1.0 / np.sqrt(np.maximum(1e-300, 1.0 - ratio ** 2)) - 1.0),
+                  0.0)
+    return (-1.0 if reference_formula else 1.0) * dr / pixel
+
+
+def rcmc_weights(frac):
+    """The kernel's 8 interpolation weights for taps k = -3 .. 4 at
+    fractional offset frac (rda_kernels.metal:163-179): sinc(frac - k) (1
+    within 1e-6 of 0) times the Hamming window 0.54 - 0.46 cos(2 pi (k+3)/7)."""
+    frac = np.asarray(frac, np.float64)
+    out = []
+    for k in range(-3, 5):
+        x = frac - k
+        sinc = np.where(np.abs(x) < 1e-6, 1.0, np.sin(np.pi * x) / np.where(x == 0, 1.0, np.pi * x))
+        out.append(sinc * (0.54 - 0.46 * np.cos(2.0 * np.pi * (k + 3) / 7.0)))
+    return np.stack(out)  # (8, ...)
+
+
+def f64_rcmc(rd, p, reference_formula=False):
+    """RCMC on range-Doppler data laid out range-bin-major (Nr x Na, the
+    layout after the corner turn and azimuth FFT): out[r, f] = sum over
+    k = -3..4 of rd[r + c(f) + k, f] w_k(frac(f)), shift(f) = c(f) + frac(f)
+    (the same for every range bin), taps outside [0, Nr) dropped
+    (rda_kernels.metal:150-180)."""
+    return rcmc_apply(rd, sar_rcmc_shift(p, reference_formula))
+
+
+def rcmc_apply(rd, shift):
+    """The interpolation of f64_rcmc for an explicit per-column shift."""
+    rd = np.asarray(rd, np.complex128)
+    nr, na = rd.shape
+    c = np.floor(shift)
+    w = rcmc_weights(shift - c)  # (8, Na)
+    c = c.astype(np.int64)
+    out = np.zeros_like(rd)
+    rows = np.arange(nr)[:, None]
+    cols = np.arange(na)[None, :]
+    for i, k in enumerate(range(-3, 5)):
+        idx = rows + c[None, :] + k
+        ok = (idx >= 0) & (idx < nr)
+        out += np.where(ok, rd[np.clip(idx, 0, nr - 1), cols], 0) * w[i][None, :]
+    return out
+
+
+def f64_rda(raw, p, range_scale=1.0, azimuth_scale=1.0, reference_formula=False, rcmc=False):
+    """Range compression -> corner turn -> azimuth FFT -> (RCMC) -> azimuth
+    filter -> azimuth IFFT in FP64; image is range-bin-major (Nr x Na)."""
     chirp, _ = sar_chirp_reference(p)
     hr = np.conj(np.fft.fft(chirp))
     rc = np.fft.ifft(np.fft.fft(np.asarray(raw, np.complex128), axis=1) * hr, axis=1) * range_scale
     ct = np.ascontiguousarray(rc.T)
     del rc
+    rd = np.fft.fft(ct, axis=1)
+    del ct
+    if rcmc:
+        rd = f64_rcmc(rd, p, reference_formula)
     ha = sar_azimuth_filter(p, reference_formula)
-    return np.fft.ifft(np.fft.fft(ct, axis=1) * ha, axis=1) * azimuth_scale
+    return np.fft.ifft(rd * ha, axis=1) * azimuth_scale
 
 
 def sar_targets(p, count, seed):

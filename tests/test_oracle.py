@@ -220,3 +220,52 @@ def test_fp16_emulation_mode_ordering(oracle):
     e = {m: oracle.rel_l2(oracle.fused_convolve_fp16_emulated(x, k, m), ref)
          for m in ("pure", "storage", "mixed")}
     assert e["storage"] < e["mixed"] and e["storage"] < e["pure"]
+
+
+# ------------------------------------------------------------------- RCMC
+def test_rcmc_weights_closed_form(oracle):
+    """At zero fraction the windowed sinc is the centre tap alone, scaled by
+    the Hamming window at (k+3)/7 = 3/7 (rda_kernels.metal:163-179); at
+    one half the taps are symmetric about k = 0.5."""
+    w = oracle.rcmc_weights(np.array(0.0))
+    expect = np.zeros(8)
+    expect[3] = 0.54 - 0.46 * np.cos(2 * np.pi * 3 / 7)
+    assert np.allclose(w, expect, atol=1e-15)
+    h = oracle.rcmc_weights(np.array(0.5))
+    win = 0.54 - 0.46 * np.cos(2 * np.pi * np.arange(8) / 7)
+    sinc = np.sinc(0.5 - np.arange(-3, 5))
+    assert np.allclose(h, sinc * win, atol=1e-15)
+
+
+def test_rcmc_integer_shift_is_scaled_copy(oracle):
+    """An integer shift s reads row r + s: a scaled copy, zero past the edge."""
+    rng = np.random.default_rng(3)
+    rd = rng.standard_normal((40, 6)) + 1j * rng.standard_normal((40, 6))
+    shift = np.array([0.0, 1.0, -2.0, 5.0, -7.0, 3.0])
+    out = oracle.rcmc_apply(rd, shift)
+    w3 = 0.54 - 0.46 * np.cos(2 * np.pi * 3 / 7)
+    for f, s in enumerate(shift.astype(int)):
+        for r in range(40):
+            src = r + s
+            want = w3 * rd[src, f] if 0 <= src < 40 else 0.0
+            assert abs(out[r, f] - want) < 1e-12
+
+
+def test_rcmc_matched_sign_focuses(oracle):
+    """Physics pin for the shift formula and its sign: with large range
+    migration (L band, 200 m/s, 5 km) a point target focuses to a higher
+    peak, on the predicted pixel, with RCMC reading r + shift; the
+    reference's literal r - shift defocuses it below no correction."""
+    p = dict(oracle.sar_default_params())
+    p.update(n_range=1024, n_azimuth=2048, carrier_freq=1e9, velocity=200.0, center_range=5000.0)
+    assert np.abs(oracle.sar_rcmc_shift(p)).max() > 100  # bins at the Doppler edge
+    raw = oracle.sar_simulate(p, [(5000.0, 0.0, 1.0 + 0j)])
+    peaks = {}
+    for key, kw in (("none", {}), ("matched", dict(rcmc=True)),
+                    ("reference", dict(rcmc=True, reference_formula=True))):
+        img = np.abs(oracle.f64_rda(raw, p, **kw))
+        peaks[key] = (img.max(), np.unravel_index(np.argmax(img), img.shape))
+    rb, ab = oracle.sar_expected_bins(p, 5000.0, 0.0)
+    assert peaks["matched"][0] > 1.5 * peaks["none"][0]
+    assert peaks["matched"][1] == (int(round(rb)), int(round(ab)))
+    assert peaks["reference"][0] < peaks["none"][0]

@@ -1,20 +1,22 @@
 """SAR range-Doppler focusing (range compression -> corner turn -> azimuth
-compression) on the GPU vs the FP64 oracle pipeline on the same raw data,
-plus target peak positions."""
+FFT -> RCMC -> azimuth filter -> IFFT) on the GPU vs the FP64 oracle pipeline
+on the same raw data, plus target peak positions."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def run_rda(gpu, oracle, p, targets, io, noise=0.1, seed=7, convention=0):
+def run_rda(gpu, oracle, p, targets, io, noise=0.1, seed=7, convention=0, rcmc=1):
     import torch
     from metalfft_amd.sar import RdaParams, RangeDoppler
     raw = oracle.sar_simulate(p, targets)
     rng = np.random.default_rng(seed)
     raw += noise * (rng.standard_normal(raw.shape) + 1j * rng.standard_normal(raw.shape))
+    geo = {k: p[k] for k in ("bandwidth", "pulse_duration", "sampling_rate", "prf", "velocity",
+                             "carrier_freq", "center_range")}
     prm = RdaParams.defaults(n_range=p["n_range"], n_azimuth=p["n_azimuth"], io=io,
-                             filter_convention=convention)
+                             filter_convention=convention, rcmc=rcmc, **geo)
     rda = RangeDoppler(prm)
     ri = raw.astype(np.complex64).view(np.float32)
     if io == 1:
@@ -28,16 +30,16 @@ def run_rda(gpu, oracle, p, targets, io, noise=0.1, seed=7, convention=0):
     torch.cuda.synchronize()
     got = img.float().cpu().numpy().view(np.complex64)
     ref = oracle.f64_rda(raw_used, p, rda.range_scale, rda.azimuth_scale,
-                         reference_formula=bool(convention))
+                         reference_formula=bool(convention), rcmc=bool(rcmc))
     return got, ref, rda
 
 
-@pytest.mark.parametrize("io,bound", [(0, 1e-5), (1, 3e-3)])
-def test_rda_small_vs_fp64(gpu, oracle, io, bound):
+@pytest.mark.parametrize("io,bound,rcmc", [(0, 1e-5, 0), (1, 3e-3, 0), (0, 1e-5, 1), (1, 3e-3, 1)])
+def test_rda_small_vs_fp64(gpu, oracle, io, bound, rcmc):
     p = oracle.sar_default_params()
     p.update(n_range=2048, n_azimuth=1024)
     tg = oracle.sar_targets(p, 8, 5)
-    got, ref, _ = run_rda(gpu, oracle, p, tg, io)
+    got, ref, _ = run_rda(gpu, oracle, p, tg, io, rcmc=rcmc)
     err = oracle.rel_l2(got, ref)
     print(f"io={io} rel_l2={err:.3e}")
     assert err < bound
@@ -52,12 +54,32 @@ def test_rda_small_vs_fp64(gpu, oracle, io, bound):
         assert abs(ri - 3 + m[0] - rb) <= 1.0 and abs(ai - 8 + m[1] - ab) <= 4.0
 
 
-def test_rda_reference_formula_matches_oracle(gpu, oracle):
+@pytest.mark.parametrize("rcmc", [0, 1])
+def test_rda_reference_formula_matches_oracle(gpu, oracle, rcmc):
     p = oracle.sar_default_params()
     p.update(n_range=2048, n_azimuth=512)
     tg = oracle.sar_targets(p, 4, 6)
-    got, ref, _ = run_rda(gpu, oracle, p, tg, 0, convention=1)
+    got, ref, _ = run_rda(gpu, oracle, p, tg, 0, convention=1, rcmc=rcmc)
     assert oracle.rel_l2(got, ref) < 1e-5
+
+
+@pytest.mark.parametrize("io,bound", [(0, 1e-5), (1, 3e-3)])
+def test_rcmc_large_migration(gpu, oracle, io, bound):
+    """L band, 200 m/s, 5 km: up to ~300 range bins of migration across the
+    Doppler band; the GPU pipeline with RCMC matches the FP64 one and
+    focuses the target on its pixel."""
+    p = dict(oracle.sar_default_params())
+    p.update(n_range=1024, n_azimuth=2048, carrier_freq=1e9, velocity=200.0, center_range=5000.0)
+    tg = [(5000.0, 0.0, 1.0 + 0j), (4980.0, 30.0, np.exp(1j))]
+    got, ref, _ = run_rda(gpu, oracle, p, tg, io, noise=0.01)
+    assert oracle.rel_l2(got, ref) < bound
+    mag = np.abs(got)
+    for r0, xt, _a in tg:
+        rb, ab = oracle.sar_expected_bins(p, r0, xt)
+        ri, ai = int(round(rb)), int(round(ab))
+        w = mag[ri - 6:ri + 7, ai - 12:ai + 13]
+        m = np.unravel_index(np.argmax(w), w.shape)
+        assert abs(ri - 6 + m[0] - rb) <= 1.0 and abs(ai - 12 + m[1] - ab) <= 2.0
 
 
 def test_rda_steps_compose(gpu, oracle):
