@@ -345,6 +345,29 @@ def extra_sar(m, torch, n=4096):
     return out
 
 
+def extra_xspec(m, torch, c=8, s=1 << 20, n=1024, hop=512):
+    """Cross-spectral analysis (PSD.crossSpectral): 8 channels x 2^20 samples,
+    Hann window, N = 1024, hop 512 (2047 frames, 28 pairs)."""
+    g = torch.Generator(device="cuda").manual_seed(9)
+    x = torch.randn((c, s), generator=g, device="cuda")
+    w = torch.from_numpy(m.window_coefficients("hann", n)).cuda()
+    an = m.CrossSpectral(n, c)
+    p = an.pairs
+    pw = torch.empty((c, n), device="cuda")
+    cr = torch.empty((p, n), dtype=torch.complex64, device="cuda")
+    co = torch.empty((p, n), device="cuda")
+    ms_ = torch.empty((p, n), device="cuda")
+    ms = time_fn(lambda: an.compute_device(x, s, hop, w, pw, cr, co, ms_), 10, 2, torch)
+    frames = an.frames(s, hop)
+    an.close()
+    # algorithmic bytes: frames of every channel read once (overlapping
+    # frames re-read), spectra written and read back once
+    nbytes = 4 * c * frames * n + 2 * 8 * c * frames * n
+    return {"channels": c, "samples": s, "fft": n, "hop": hop, "frames": frames, "pairs": p,
+            "ms": ms, "frames_per_s": c * frames / (ms * 1e-3), "gbs_alg": nbytes / (ms * 1e-3) / 1e9,
+            "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     try:
@@ -437,7 +460,8 @@ def main():
                         ("ragged_10k", lambda: extra_ragged(m, torch, cpu=not a.no_cpu)),
                         ("convolve_512x65536_k1023",
                          lambda: extra_convolve(m, torch, cpu=not a.no_cpu)),
-                        ("sar_4096x4096_fp16", lambda: extra_sar(m, torch))):
+                        ("sar_4096x4096_fp16", lambda: extra_sar(m, torch)),
+                        ("cross_spectral_8ch", lambda: extra_xspec(m, torch))):
             try:
                 extras[key] = fn()
             except Exception as e:  # keep the headline line whatever happens here

@@ -312,6 +312,40 @@ MIFFT_API mifft_status mifft_rda_corner_turn_device(const mifft_rda *rda, const 
 MIFFT_API mifft_status mifft_rda_azimuth_compress_device(const mifft_rda *rda, const void *in,
                                                          void *out, void *stream);
 
+/* ------------------------------------------------------- cross-spectral */
+typedef struct mifft_xspec mifft_xspec;
+
+/* Analyser for PSD.crossSpectral(channels:fftSize:hopSize:sampleRate:window:)
+ * (Sources/MetalFFT/DSP/CrossSpectral.swift:55-173): `channels` (1..16, else
+ * MIFFT_ERR_PRECONDITION; the reference traps) signals, frames of fft_size
+ * (a supported FFT size). */
+MIFFT_API mifft_status mifft_xspec_create(int64_t fft_size, int64_t channels, mifft_xspec **x);
+MIFFT_API mifft_status mifft_xspec_destroy(mifft_xspec *x);
+/* Frames the analysis uses: pos = 0, hop, 2 hop, ... while pos + fft_size <=
+ * samples (CrossSpectral.swift:95-96). */
+MIFFT_API int64_t mifft_xspec_frames(const mifft_xspec *x, int64_t samples, int64_t hop);
+/* Device data: signals[channels][samples] (FP32, rows back to back), window
+ * [fft_size] multiplied into every frame (null: rectangular; the reference's
+ * default is .hann, Window.swift:32-36), hop >= 1.  Outputs, each averaged
+ * over the frames (CrossSpectralResult, CrossSpectral.swift:11-39;
+ * fft_cross_spectral.metal:29-64):
+ *   power[channels][fft_size]        |X_c(k)|^2
+ *   cross[P][fft_size] (complex)     X_i(k) conj(X_j(k)), pairs (0,1), (0,2),
+ *                                    ..., (C-2, C-1), P = C (C-1) / 2
+ *   coherence[P][fft_size]           per-frame |S_ij|^2 / max(P_i P_j, 1e-30),
+ *                                    averaged (the reference's definition: 1
+ *                                    wherever both channels have power)
+ *   welch_msc[P][fft_size] (may be null)  |<S_ij>|^2 / max(<P_i><P_j>, 1e-30),
+ *                                    the coherence of the averaged spectra
+ *                                    (no counterpart in the reference).
+ * No frame (samples < fft_size): all outputs zero, as the reference's
+ * accumulators.  Asynchronous on `stream`; scratch is analyser-owned, so
+ * calls on one analyser must not overlap. */
+MIFFT_API mifft_status mifft_xspec_device(mifft_xspec *x, const float *signals, int64_t samples,
+                                          int64_t hop, const float *window, float *power,
+                                          float *cross, float *coherence, float *welch_msc,
+                                          void *stream);
+
 /* ------------------------------------------------------------ diagnostic */
 /* No counterpart in the reference.  Copies `bytes` from src to dst (device
  * memory, not overlapping) with the access style of the FFT kernels

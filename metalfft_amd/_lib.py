@@ -111,6 +111,11 @@ _SIGNATURES = [
     ("mifft_correlator_cross_device", c_int,
      [c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p, c_void_p]),
     ("mifft_copy_device", c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    ("mifft_xspec_create", c_int, [c_int64, c_int64, ctypes.POINTER(c_void_p)]),
+    ("mifft_xspec_destroy", c_int, [c_void_p]),
+    ("mifft_xspec_frames", c_int64, [c_void_p, c_int64, c_int64]),
+    ("mifft_xspec_device", c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p]),
 ]
 
 _lib = None

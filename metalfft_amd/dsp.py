@@ -5,6 +5,9 @@ libmifft C-ABI.
     FusedConvolver(kernel, fft_size=4096,  FusedConvolver.swift:17-134
                    precision=Precision.float32)
     Correlator(fft_size)                   Correlate.swift:7-62
+    cross_spectral(channels, fft_size,     CrossSpectral.swift:55-173
+                   hop_size, window="hann")
+    CrossSpectral(fft_size, channels)      (device form of the same)
 
 ``apply`` / ``cross`` / ``auto`` take host arrays and block, as the Swift
 methods do.  ``apply_device`` / ``backward_device`` / ``cross_device`` run
@@ -141,4 +144,115 @@ class Correlator:
             pass
 
 
-__all__ = ["Convolver", "FusedConvolver", "Correlator", "Precision", "FFTError"]
+def window_coefficients(kind: str, n: int) -> np.ndarray:
+    """WindowType.coefficients (Sources/MetalFFT/DSP/Window.swift:14-23) for
+    the vDSP windows it calls, from their published definitions (periodic,
+    denominator N): hann 0.5 (1 - cos 2 pi n/N) (vDSP_HANN_NORM), hamming
+    0.54 - 0.46 cos 2 pi n/N, blackman 0.42 - 0.5 cos 2 pi n/N + 0.08 cos
+    4 pi n/N, rectangular 1."""
+    k = np.arange(n, dtype=np.float64)
+    if kind == "hann":
+        w = 0.5 * (1.0 - np.cos(2.0 * np.pi * k / n))
+    elif kind == "hamming":
+        w = 0.54 - 0.46 * np.cos(2.0 * np.pi * k / n)
+    elif kind == "blackman":
+        w = 0.42 - 0.5 * np.cos(2.0 * np.pi * k / n) + 0.08 * np.cos(4.0 * np.pi * k / n)
+    elif kind == "rectangular":
+        w = np.ones(n)
+    else:
+        raise ValueError(f"unknown window {kind!r}")
+    return w.astype(np.float32)
+
+
+class CrossSpectralResult:
+    """CrossSpectralResult (CrossSpectral.swift:11-39): power[C][N],
+    crossSpectra[P][N] (complex), coherence[P][N]; welch_msc[P][N] is the
+    coherence of the averaged spectra (no counterpart in the reference)."""
+
+    def __init__(self, channels, fft_size, power, cross, coherence, welch_msc):
+        self.channels = channels
+        self.fftSize = fft_size
+        self.power = power
+        self.crossSpectra = cross
+        self.coherence = coherence
+        self.welch_msc = welch_msc
+
+    @property
+    def pairCount(self) -> int:
+        return self.channels * (self.channels - 1) // 2
+
+    def pairIndex(self, i: int, j: int) -> int:
+        if not (0 <= i < j < self.channels):
+            raise FFTError(10, "pairIndex requires i < j < channels")
+        return sum(self.channels - 1 - a for a in range(i)) + (j - i - 1)
+
+
+class CrossSpectral:
+    """Device-side cross-spectral analyser over C channels of FFT size N
+    (mifft_xspec_*): ``compute_device`` takes a (C, samples) float32 CUDA
+    tensor and writes the averaged outputs into CUDA tensors."""
+
+    def __init__(self, fft_size: int, channels: int):
+        self._x = ctypes.c_void_p()
+        _lib.check(_lib.load().mifft_xspec_create(int(fft_size), int(channels), ctypes.byref(self._x)))
+        self.fftSize, self.channels = int(fft_size), int(channels)
+        self.pairs = self.channels * (self.channels - 1) // 2
+
+    def frames(self, samples: int, hop: int) -> int:
+        return int(_lib.load().mifft_xspec_frames(self._x, int(samples), int(hop)))
+
+    def compute_device(self, signals, samples: int, hop: int, window, power, cross, coherence,
+                       welch_msc=None, stream=None) -> None:
+        _lib.check(_lib.load().mifft_xspec_device(
+            self._x, _dev_ptr(signals) if signals is not None else None, int(samples), int(hop),
+            _dev_ptr(window) if window is not None else None, _dev_ptr(power),
+            _dev_ptr(cross) if cross is not None else None,
+            _dev_ptr(coherence) if coherence is not None else None,
+            _dev_ptr(welch_msc) if welch_msc is not None else None, _stream_handle(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_x", None) and self._x.value:
+            _lib.load().mifft_xspec_destroy(self._x)
+            self._x = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cross_spectral(channels, fft_size: int, hop_size: int, sample_rate: float = 0.0,
+                   window="hann") -> CrossSpectralResult:
+    """PSD.crossSpectral(channels:fftSize:hopSize:sampleRate:window:)
+    (CrossSpectral.swift:55-173) on host arrays (sample_rate labels bins only
+    and is unused, as in the reference).  Raises FFTError
+    (MIFFT_ERR_PRECONDITION) where the reference traps."""
+    import torch
+    sig = [np.asarray(c, dtype=np.float32).reshape(-1) for c in channels]
+    if not sig or len(sig) > 16:
+        raise FFTError(10, "crossSpectral requires 1-16 channels")
+    if any(s.size != sig[0].size for s in sig):
+        raise FFTError(10, "all channels must have the same length")
+    C, S, N = len(sig), sig[0].size, int(fft_size)
+    an = CrossSpectral(N, C)
+    try:
+        P = an.pairs
+        x = torch.from_numpy(np.stack(sig)).cuda()
+        w = None if window is None else torch.from_numpy(
+            window_coefficients(window, N) if isinstance(window, str)
+            else np.asarray(window, np.float32)).cuda()
+        pw = torch.empty((C, N), dtype=torch.float32, device="cuda")
+        cr = torch.empty((max(P, 1), N), dtype=torch.complex64, device="cuda")
+        co = torch.empty((max(P, 1), N), dtype=torch.float32, device="cuda")
+        ms = torch.empty((max(P, 1), N), dtype=torch.float32, device="cuda")
+        an.compute_device(x, S, hop_size, w, pw, cr, co, ms)
+        torch.cuda.synchronize()
+        return CrossSpectralResult(C, N, pw.cpu().numpy(), cr[:P].cpu().numpy(),
+                                   co[:P].cpu().numpy(), ms[:P].cpu().numpy())
+    finally:
+        an.close()
+
+
+__all__ = ["Convolver", "FusedConvolver", "Correlator", "Precision", "FFTError", "CrossSpectral",
+           "CrossSpectralResult", "cross_spectral", "window_coefficients"]

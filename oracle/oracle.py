@@ -250,6 +250,42 @@ def fused_convolve_fp16_emulated(signal: np.ndarray, taps: np.ndarray, mode: str
     return out
 
 
+# ------------------------------------------------ cross-spectral (FP64)
+def f64_cross_spectral(channels, fft_size, hop, window=None):
+    """PSD.crossSpectral (Sources/MetalFFT/DSP/CrossSpectral.swift:55-173) in
+    FP64: frames at pos = 0, hop, ... while pos + N <= samples, each channel
+    windowed and transformed; per frame |X_c|^2, S_ij = X_i conj(X_j) over the
+    upper-triangle pairs (0,1), (0,2), ..., and |S_ij|^2 / max(P_i P_j,
+    1e-30) (fft_cross_spectral.metal:29-64), all averaged over frames; plus
+    the coherence of the averaged spectra.  Returns (power, cross, coherence,
+    welch_msc); zeros when there is no frame."""
+    x = np.asarray(channels, np.float64)
+    c, s = x.shape
+    n = int(fft_size)
+    w = np.ones(n) if window is None else np.asarray(window, np.float64)
+    pairs = [(i, j) for i in range(c) for j in range(i + 1, c)]
+    starts = list(range(0, s - n + 1, hop)) if s >= n else []
+    power = np.zeros((c, n))
+    cross = np.zeros((len(pairs), n), np.complex128)
+    coh = np.zeros((len(pairs), n))
+    for pos in starts:
+        X = np.fft.fft(x[:, pos:pos + n] * w, axis=1)
+        P = np.abs(X) ** 2
+        power += P
+        for q, (i, j) in enumerate(pairs):
+            S = X[i] * np.conj(X[j])
+            cross[q] += S
+            coh[q] += np.abs(S) ** 2 / np.maximum(P[i] * P[j], 1e-30)
+    f = max(1, len(starts))
+    power, cross, coh = power / f, cross / f, coh / f
+    msc = np.zeros_like(coh)
+    for q, (i, j) in enumerate(pairs):
+        msc[q] = np.abs(cross[q]) ** 2 / np.maximum(power[i] * power[j], 1e-30)
+    if not starts:
+        msc[:] = 0
+    return power, cross, coh, msc
+
+
 # ---------------------------------------------------------------- metrics
 def rel_l2(a, b) -> float:
     """sqrt(sum|a-b|^2 / sum|b|^2), Tests/MetalFFTTests/MetalFFTTests.swift:190-198."""

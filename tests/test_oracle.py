@@ -269,3 +269,36 @@ def test_rcmc_matched_sign_focuses(oracle):
     assert peaks["matched"][0] > 1.5 * peaks["none"][0]
     assert peaks["matched"][1] == (int(round(rb)), int(round(ab)))
     assert peaks["reference"][0] < peaks["none"][0]
+
+
+# -------------------------------------------------------- cross-spectral
+def test_cross_spectral_closed_forms(oracle):
+    """A cosine with an integer number of cycles per frame (rectangular
+    window) has power (A N / 2)^2 at its bin; identical channels give
+    cross == power and coherence 1; pairs run (0,1), (0,2), (1,2)."""
+    n, hop = 256, 128
+    t = np.arange(4 * n)
+    a = 0.75 * np.cos(2 * np.pi * 10 * t / n)
+    pw, cr, co, ms = oracle.f64_cross_spectral([a, a, 2 * a], n, hop)
+    assert abs(pw[0, 10] - (0.75 * n / 2) ** 2) < 1e-6 * pw[0, 10]
+    assert np.allclose(cr[0], pw[0], rtol=1e-12, atol=1e-9)
+    assert np.allclose(cr[1], 2 * pw[0], rtol=1e-12, atol=1e-9)  # (0, 2)
+    assert np.allclose(cr[2], 2 * pw[0], rtol=1e-12, atol=1e-9)  # (1, 2)
+    big = pw[0] > 1e-6 * pw[0].max()
+    assert np.allclose(co[:, big], 1.0) and np.allclose(ms[:, big], 1.0)
+
+
+def test_cross_spectral_frames_and_independence(oracle):
+    """Frame count (pos + N <= samples); independent white noise: the
+    per-frame coherence is 1 (the reference's definition) while the
+    coherence of averaged spectra falls to ~1/frames."""
+    rng = np.random.default_rng(4)
+    n, hop = 128, 64
+    x = rng.standard_normal((2, 64 * 64 + 10))
+    pw, cr, co, ms = oracle.f64_cross_spectral(x, n, hop)
+    frames = (x.shape[1] - n) // hop + 1
+    assert frames == 63
+    assert np.allclose(co, 1.0)
+    assert 0.2 / frames < ms.mean() < 3.0 / frames
+    none = oracle.f64_cross_spectral(x[:, :n - 1], n, hop)
+    assert all(not np.any(a) for a in none)
