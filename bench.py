@@ -368,6 +368,53 @@ def extra_xspec(m, torch, c=8, s=1 << 20, n=1024, hop=512):
             "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
+def extra_real(m, torch, n=4096, batch=4096):
+    """Real-input transforms, 4096 real signals of 4096 samples: forward to
+    all N bins, to bins 0..N/2, and the inverse from bins 0..N/2."""
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.randn((batch, n), generator=g, device="cuda")
+    full = torch.empty((batch, n), dtype=torch.complex64, device="cuda")
+    half = torch.empty((batch, n // 2 + 1), dtype=torch.complex64, device="cuda")
+    back = torch.empty_like(x)
+    f = m.MetalFFT(n)
+    out = {"fft": n, "batch": batch}
+    for key, fn, nbytes in (
+            ("forward_full", lambda: f.forward_real(x, full, batch), (4 * n + 8 * n) * batch),
+            ("forward_half", lambda: f.forward_real(x, half, batch, half=True),
+             (4 * n + 8 * (n // 2 + 1)) * batch),
+            ("inverse_half", lambda: f.inverse_real(half, back, batch, half=True),
+             (4 * n + 8 * (n // 2 + 1)) * batch)):
+        ms = time_fn(fn, 20, 3, torch)
+        out[key] = {"ms": ms, "transforms_per_s": batch / (ms * 1e-3),
+                    "gbs_alg": nbytes / (ms * 1e-3) / 1e9,
+                    "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
+def extra_stft(m, torch, s=1 << 24, n=1024, hop=256):
+    """Batched framing: STFT.analyze of one 2^24-sample signal, Hann window,
+    N = 1024, hop 256 (65533 frames in one launch; spectra and |X|^2
+    written), and PSD.welch of the same signal."""
+    g = torch.Generator(device="cuda").manual_seed(10)
+    x = torch.randn((1, s), generator=g, device="cuda")
+    w = torch.from_numpy(m.window_coefficients("hann", n)).cuda()
+    an = m.CrossSpectral(n, 1)
+    f = an.frames(s, hop)
+    spec = torch.empty((f, n), dtype=torch.complex64, device="cuda")
+    pw = torch.empty((f, n), device="cuda")
+    psd = torch.empty(n, device="cuda")
+    ms = time_fn(lambda: an.stft_device(x, s, hop, w, spec, pw), 10, 2, torch)
+    ms_w = time_fn(lambda: an.welch_device(x, s, hop, w, 44100.0, psd), 10, 2, torch)
+    an.close()
+    # algorithmic bytes: the signal read once, spectra (8 B) and |X|^2 (4 B)
+    # per bin written
+    nbytes = 4 * s + 12 * f * n
+    return {"samples": s, "fft": n, "hop": hop, "frames": f, "stft_ms": ms,
+            "frames_per_s": f / (ms * 1e-3), "gbs_alg": nbytes / (ms * 1e-3) / 1e9,
+            "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "welch_ms": ms_w,
+            "welch_frames_per_s": f / (ms_w * 1e-3)}
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     try:
@@ -461,7 +508,9 @@ def main():
                         ("convolve_512x65536_k1023",
                          lambda: extra_convolve(m, torch, cpu=not a.no_cpu)),
                         ("sar_4096x4096_fp16", lambda: extra_sar(m, torch)),
-                        ("cross_spectral_8ch", lambda: extra_xspec(m, torch))):
+                        ("cross_spectral_8ch", lambda: extra_xspec(m, torch)),
+                        ("stft_welch_2p24", lambda: extra_stft(m, torch)),
+                        ("real_fft_4096x4096", lambda: extra_real(m, torch))):
             try:
                 extras[key] = fn()
             except Exception as e:  # keep the headline line whatever happens here

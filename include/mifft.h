@@ -133,6 +133,26 @@ MIFFT_API mifft_status mifft_plan_select_kernel(mifft_plan *plan, const char *na
 MIFFT_API mifft_status mifft_execute_device(const mifft_plan *plan, const float *in, float *out,
                                   int64_t batch, mifft_direction dir, void *stream);
 
+/* Real-input transforms of `batch` contiguous length-n float signals,
+ * asynchronous on `stream`.  The reference's callers pack each real sample
+ * as (x, 0) and run the complex transform (Convolve.swift:48-53,
+ * FusedConvolver.swift:101-107, Correlate.swift:53-61, STFT.swift:116-122,
+ * PSD.swift:37-45, FFTAnalyzer.swift:93-99); this computes the same spectra
+ * with an n/2-point complex FFT of (x[2j], x[2j+1]) and a split, half the
+ * work.  half = 0: y holds all n bins per signal (complex64, Hermitian: what
+ * forward(input:output:) returns for the packed input); half = 1: bins
+ * 0..n/2 (n/2 + 1 complex64 per signal).  x and y must not overlap.  FP32
+ * plans only. */
+MIFFT_API mifft_status mifft_forward_real_device(const mifft_plan *plan, const float *x, float *y,
+                                                 int64_t batch, int half, void *stream);
+/* Inverse of Hermitian spectra to real signals: y rows of n (half = 0) or
+ * n/2 + 1 (half = 1) complex64, of which bins 0..n/2 are read;
+ * x[b * n + j] = (1/n) sum_k X[k] exp(+2 pi i j k / n), the real part of
+ * MetalFFT.inverse (MetalFFT.swift:56-65) that Convolve.swift:70-75 and
+ * FusedConvolver keep.  x and y must not overlap. */
+MIFFT_API mifft_status mifft_inverse_real_device(const mifft_plan *plan, const float *y, int half,
+                                                 float *x, int64_t batch, void *stream);
+
 /* MetalFFT.forward(_:) (MetalFFT.swift:69-85) and forward(input:output:)
  * (:88-104): in_count must equal n (else MIFFT_ERR_INVALID_INPUT_SIZE with
  * expected/got); `out` holds n complex values.  Blocking. */
@@ -345,6 +365,31 @@ MIFFT_API mifft_status mifft_xspec_device(mifft_xspec *x, const float *signals, 
                                           int64_t hop, const float *window, float *power,
                                           float *cross, float *coherence, float *welch_msc,
                                           void *stream);
+
+/* Batched framing for STFT.analyze (Sources/MetalFFT/DSP/STFT.swift:70-99,
+ * 116-128): every frame of every channel (frames as mifft_xspec_frames)
+ * windowed and transformed in one launch instead of one synchronous FFT per
+ * frame.  spectra[channels][frames][fft_size] complex64 (STFTFrame.complex);
+ * power (may be null) |X|^2 (STFTFrame.magnitudes, Spectrum.swift:25-31);
+ * power_db (may be null) db_alpha * log10(|X|^2) clipped below at -120 dB
+ * (STFTFrame.magnitudesDB, Spectrum.toDecibels, Spectrum.swift:47-55).
+ * toDecibels calls vDSP_vdbcon with flag 1, whose factor is 20 (amplitude
+ * convention), although its comment says 10 log10: db_alpha = 20 gives what
+ * the reference computes, 10 what its comment describes.  FFTAnalyzer.analyze
+ * (FFTAnalyzer.swift:92-116) is the one-frame case (samples = fft_size).  No
+ * frame: nothing is written.  Asynchronous on `stream`. */
+MIFFT_API mifft_status mifft_stft_device(mifft_xspec *x, const float *signals, int64_t samples,
+                                         int64_t hop, const float *window, float *spectra,
+                                         float *power, float *power_db, float db_alpha,
+                                         void *stream);
+/* PSD.welch (Sources/MetalFFT/DSP/PSD.swift:18-59) per channel:
+ * psd[channels][fft_size] = mean over frames of |X|^2 / (sum of window^2 *
+ * sample_rate) (sample_rate > 0, else MIFFT_ERR_PRECONDITION); zeros when
+ * there is no frame.  PSD.coherence (PSD.swift:72-125) is welch_msc of a
+ * two-channel mifft_xspec_device.  Asynchronous on `stream`. */
+MIFFT_API mifft_status mifft_welch_device(mifft_xspec *x, const float *signals, int64_t samples,
+                                          int64_t hop, const float *window, double sample_rate,
+                                          float *psd, void *stream);
 
 /* ------------------------------------------------------------ diagnostic */
 /* No counterpart in the reference.  Copies `bytes` from src to dst (device

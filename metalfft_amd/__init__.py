@@ -8,12 +8,14 @@ from ._lib import FFTError, FORWARD, INVERSE  # noqa: F401
 from .api import (MetalFFT, MultiChannelFFT, PrecisionFFT, RaggedFFT,  # noqa: F401
                   RAGGED_GRAPH, RAGGED_PER_SIZE, RAGGED_PERSISTENT, SUPPORTED_SIZES,
                   device_name, kernel_variants)
-from .dsp import (Convolver, Correlator, CrossSpectral, CrossSpectralResult,  # noqa: F401
-                  FusedConvolver, Precision, cross_spectral, window_coefficients)
+from .dsp import (PSD, STFT, AnalysisResult, Convolver, Correlator,  # noqa: F401
+                  CrossSpectral, CrossSpectralResult, FFTAnalyzer, FusedConvolver, Precision,
+                  STFTFrame, cross_spectral, to_decibels, window_coefficients)
 from .sar import RangeDoppler, RdaParams  # noqa: F401
 
 __all__ = ["FFTError", "FORWARD", "INVERSE", "MetalFFT", "MultiChannelFFT",
            "SUPPORTED_SIZES", "device_name", "kernel_variants", "Convolver", "FusedConvolver",
            "Correlator", "Precision", "RaggedFFT", "PrecisionFFT", "RAGGED_GRAPH",
            "RAGGED_PER_SIZE", "RAGGED_PERSISTENT", "RangeDoppler", "RdaParams", "CrossSpectral",
-           "CrossSpectralResult", "cross_spectral", "window_coefficients"]
+           "CrossSpectralResult", "cross_spectral", "window_coefficients", "STFT", "STFTFrame",
+           "FFTAnalyzer", "AnalysisResult", "PSD", "to_decibels"]

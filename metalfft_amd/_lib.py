@@ -73,6 +73,8 @@ _SIGNATURES = [
     ("mifft_kernel_variants", c_int, [c_int64, ctypes.POINTER(c_char_p), c_int]),
     ("mifft_plan_select_kernel", c_int, [c_void_p, c_char_p]),
     ("mifft_execute_device", c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    ("mifft_forward_real_device", c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    ("mifft_inverse_real_device", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int64, c_void_p]),
     ("mifft_forward_host", c_int, [c_void_p, fptr, c_int64, fptr]),
     ("mifft_inverse_host", c_int, [c_void_p, fptr, c_int64, fptr]),
     ("mifft_forward_batch_host", c_int, [c_void_p, fpp, i64p, c_int64, fpp]),
@@ -116,6 +118,10 @@ _SIGNATURES = [
     ("mifft_xspec_frames", c_int64, [c_void_p, c_int64, c_int64]),
     ("mifft_xspec_device", c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("mifft_stft_device", c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, ctypes.c_float, c_void_p]),
+    ("mifft_welch_device", c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p,
+                                   ctypes.c_double, c_void_p, c_void_p]),
 ]
 
 _lib = None

@@ -169,6 +169,35 @@ class MetalFFT:
                                                     int(batch), int(direction),
                                                     _stream_handle(stream)))
 
+    def forward_real(self, src, dst, batch: int, half: bool = False, stream=None) -> None:
+        """Spectra of ``batch`` real float32 signals of length ``size`` on the
+        GPU (mifft_forward_real_device): ``dst`` holds ``size`` complex64 bins
+        per signal, or ``size // 2 + 1`` with ``half``."""
+        _lib.check(_lib.load().mifft_forward_real_device(self._plan, _dev_ptr(src), _dev_ptr(dst),
+                                                         int(batch), int(bool(half)),
+                                                         _stream_handle(stream)))
+
+    def inverse_real(self, src, dst, batch: int, half: bool = False, stream=None) -> None:
+        """Real signals from Hermitian spectra (mifft_inverse_real_device):
+        ``src`` rows of ``size`` (or ``size // 2 + 1`` with ``half``)
+        complex64, ``dst`` ``batch`` x ``size`` float32."""
+        _lib.check(_lib.load().mifft_inverse_real_device(self._plan, _dev_ptr(src), int(bool(half)),
+                                                         _dev_ptr(dst), int(batch),
+                                                         _stream_handle(stream)))
+
+    def forwardReal(self, signal) -> np.ndarray:
+        """Spectrum of one real host signal (all ``size`` bins), the result
+        of forward() on the (x, 0) packing the reference's callers build."""
+        import torch
+        x = np.ascontiguousarray(np.asarray(signal, np.float32).reshape(-1))
+        if x.size != self.size:
+            raise FFTError(6, "Invalid input size", expected=self.size, got=x.size)
+        dx = torch.from_numpy(x).cuda()
+        y = torch.empty(self.size, dtype=torch.complex64, device="cuda")
+        self.forward_real(dx, y, 1)
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
 
 class MultiChannelFFT:
     """Same-size FFTs over several channels in one launch per call

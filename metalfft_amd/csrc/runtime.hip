@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "mfma_fft.hpp"
+#include "realfft.hpp"
 #include "runtime.hpp"
 
 namespace mifft {
@@ -271,7 +272,13 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
     }
+    if ((s = check_hip(hipMalloc(&p->real_tw, sizeof(float2) * real_table_len((int)n)), "hipMalloc")) !=
+        MIFFT_OK) {
+        mifft_plan_destroy(p);
+        return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
+    }
     if ((s = check_hip(launch_twiddles(p->twiddles, (int)n, p->stream), "twiddles")) != MIFFT_OK ||
+        (s = check_hip(launch_real_table(p->real_tw, (int)n, p->stream), "real-input table")) != MIFFT_OK ||
         (s = check_hip(hipStreamSynchronize(p->stream), "hipStreamSynchronize")) != MIFFT_OK) {
         mifft_plan_destroy(p);
         return s;
@@ -291,6 +298,7 @@ mifft_status mifft_plan_destroy(mifft_plan* plan) {
     plan->in.release();
     plan->out.release();
     if (plan->twiddles) (void)hipFree(plan->twiddles);
+    if (plan->real_tw) (void)hipFree(plan->real_tw);
     if (plan->stream) (void)hipStreamDestroy(plan->stream);
     delete plan;
     return MIFFT_OK;

@@ -43,6 +43,7 @@ struct mifft_plan {
     int io = 0;         // mifft_io
     const mifft::StockhamVariant* variant = nullptr;
     float2* twiddles = nullptr;  // W_N^k, k < N, then pass rows (twiddle_table_len)
+    float2* real_tw = nullptr;   // real-input transforms (real_table_len)
     hipStream_t stream = nullptr;
     mifft::DeviceBuffer in, out;
 };

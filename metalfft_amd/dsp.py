@@ -7,7 +7,13 @@ libmifft C-ABI.
     Correlator(fft_size)                   Correlate.swift:7-62
     cross_spectral(channels, fft_size,     CrossSpectral.swift:55-173
                    hop_size, window="hann")
-    CrossSpectral(fft_size, channels)      (device form of the same)
+    CrossSpectral(fft_size, channels)      (device form of the same, plus
+                                            batched STFT and Welch)
+    STFT(fftSize, hopSize, window,         STFT.swift:20-99
+         sampleRate)
+    FFTAnalyzer(size, sampleRate, window)  FFTAnalyzer.swift:66-116
+    PSD.welch / PSD.coherence /            PSD.swift:18-134
+        PSD.crossSpectral
 
 ``apply`` / ``cross`` / ``auto`` take host arrays and block, as the Swift
 methods do.  ``apply_device`` / ``backward_device`` / ``cross_device`` run
@@ -144,13 +150,42 @@ class Correlator:
             pass
 
 
-def window_coefficients(kind: str, n: int) -> np.ndarray:
+def _bessel_i0(x: float) -> float:
+    """Modified Bessel function I0 by its power series sum ((x/2)^k / k!)^2,
+    to 30 terms or a relative term below 1e-12 (Window.swift:99-110)."""
+    result = term = 1.0
+    h = x / 2.0
+    for k in range(1, 31):
+        term *= h * h / float(k * k)
+        result += term
+        if term < 1e-12 * result:
+            break
+    return result
+
+
+def window_coefficients(kind, n: int) -> np.ndarray:
     """WindowType.coefficients (Sources/MetalFFT/DSP/Window.swift:14-23) for
     the vDSP windows it calls, from their published definitions (periodic,
     denominator N): hann 0.5 (1 - cos 2 pi n/N) (vDSP_HANN_NORM), hamming
     0.54 - 0.46 cos 2 pi n/N, blackman 0.42 - 0.5 cos 2 pi n/N + 0.08 cos
-    4 pi n/N, rectangular 1."""
+    4 pi n/N, rectangular 1; flatTop the five-term cosine sum of
+    Window.swift:52-66; ("kaiser", beta) I0(beta sqrt(1 - x^2)) / I0(beta),
+    x = (n - (N-1)/2) / ((N-1)/2) (Window.swift:69-77; beta 6 for "kaiser")."""
     k = np.arange(n, dtype=np.float64)
+    beta = 6.0
+    if isinstance(kind, tuple):
+        kind, beta = kind[0], float(kind[1])
+    if kind == "kaiser":
+        half = (n - 1) / 2.0
+        x = (k - half) / half
+        w = np.array([_bessel_i0(beta * np.sqrt(max(0.0, 1.0 - v * v))) for v in x]) / _bessel_i0(beta)
+        return w.astype(np.float32)
+    if kind == "flatTop":
+        a = (0.21557895, 0.41663158, 0.277263158, 0.083578947, 0.006947368)
+        x = k / n
+        w = (a[0] - a[1] * np.cos(2 * np.pi * x) + a[2] * np.cos(4 * np.pi * x)
+             - a[3] * np.cos(6 * np.pi * x) + a[4] * np.cos(8 * np.pi * x))
+        return w.astype(np.float32)
     if kind == "hann":
         w = 0.5 * (1.0 - np.cos(2.0 * np.pi * k / n))
     elif kind == "hamming":
@@ -210,6 +245,27 @@ class CrossSpectral:
             _dev_ptr(coherence) if coherence is not None else None,
             _dev_ptr(welch_msc) if welch_msc is not None else None, _stream_handle(stream)))
 
+    def stft_device(self, signals, samples: int, hop: int, window, spectra, power=None,
+                    power_db=None, db_alpha: float = 20.0, stream=None) -> None:
+        """All frames of all channels windowed and transformed in one launch
+        (mifft_stft_device): spectra (C, frames, N) complex64, optional
+        power |X|^2 and power_db (C, frames, N) float32."""
+        _lib.check(_lib.load().mifft_stft_device(
+            self._x, _dev_ptr(signals) if signals is not None else None, int(samples), int(hop),
+            _dev_ptr(window) if window is not None else None,
+            _dev_ptr(spectra) if spectra is not None else None,
+            _dev_ptr(power) if power is not None else None,
+            _dev_ptr(power_db) if power_db is not None else None, float(db_alpha),
+            _stream_handle(stream)))
+
+    def welch_device(self, signals, samples: int, hop: int, window, sample_rate: float, psd,
+                     stream=None) -> None:
+        """PSD.welch of every channel (mifft_welch_device): psd (C, N)."""
+        _lib.check(_lib.load().mifft_welch_device(
+            self._x, _dev_ptr(signals) if signals is not None else None, int(samples), int(hop),
+            _dev_ptr(window) if window is not None else None, float(sample_rate),
+            _dev_ptr(psd) if psd is not None else None, _stream_handle(stream)))
+
     def close(self) -> None:
         if getattr(self, "_x", None) and self._x.value:
             _lib.load().mifft_xspec_destroy(self._x)
@@ -239,9 +295,7 @@ def cross_spectral(channels, fft_size: int, hop_size: int, sample_rate: float = 
     try:
         P = an.pairs
         x = torch.from_numpy(np.stack(sig)).cuda()
-        w = None if window is None else torch.from_numpy(
-            window_coefficients(window, N) if isinstance(window, str)
-            else np.asarray(window, np.float32)).cuda()
+        w = _window(window, N)
         pw = torch.empty((C, N), dtype=torch.float32, device="cuda")
         cr = torch.empty((max(P, 1), N), dtype=torch.complex64, device="cuda")
         co = torch.empty((max(P, 1), N), dtype=torch.float32, device="cuda")
@@ -254,5 +308,201 @@ def cross_spectral(channels, fft_size: int, hop_size: int, sample_rate: float = 
         an.close()
 
 
+def _window(window, n: int):
+    """Window coefficients as a CUDA tensor (None: rectangular, no multiply)."""
+    import torch
+    if window is None:
+        return None
+    spec = isinstance(window, str) or (isinstance(window, tuple) and isinstance(window[0], str))
+    w = window_coefficients(window, n) if spec else _f32(window)
+    if w.size != n:
+        raise FFTError(6, "window length must equal fftSize", expected=n, got=w.size)
+    return torch.from_numpy(w).cuda()
+
+
+class STFTFrame:
+    """STFTFrame (Sources/MetalFFT/DSP/STFT.swift:5-15): complex spectrum,
+    magnitudes |X|^2, magnitudesDB (Spectrum.toDecibels: 20 log10 as
+    vDSP_vdbcon's flag 1 computes, floor -120) and phase atan2(im, re)."""
+
+    def __init__(self, complex_, magnitudes, magnitudes_db=None):
+        self.complex = complex_
+        self.magnitudes = magnitudes
+        self._db = magnitudes_db
+
+    @property
+    def magnitudesDB(self) -> np.ndarray:
+        if self._db is None:
+            self._db = to_decibels(self.magnitudes)
+        return self._db
+
+    @property
+    def phase(self) -> np.ndarray:
+        return np.arctan2(self.complex.imag, self.complex.real).astype(np.float32)
+
+
+def to_decibels(magnitudes, floor_db: float = -120.0, alpha: float = 20.0) -> np.ndarray:
+    """Spectrum.toDecibels (Sources/MetalFFT/DSP/Spectrum.swift:47-55) on the
+    host: alpha log10(m) clipped below at floor_db (alpha 20 is what the
+    reference's vDSP_vdbcon flag 1 computes)."""
+    m = np.asarray(magnitudes, np.float32)
+    with np.errstate(divide="ignore"):
+        return np.maximum(np.float32(alpha) * np.log10(m), np.float32(floor_db)).astype(np.float32)
+
+
+class STFT:
+    """Short-time Fourier transform (Sources/MetalFFT/DSP/STFT.swift:20-99).
+    ``analyze`` windows and transforms every frame of the signal in one
+    batched launch (the reference runs one synchronous FFT per frame) and
+    returns the same list of STFTFrame."""
+
+    def __init__(self, fftSize: int, hopSize: int, window="hann", sampleRate: float = 44100.0):
+        if int(hopSize) < 1:
+            raise FFTError(10, "hopSize must be >= 1")
+        self.fftSize, self.hopSize, self.sampleRate = int(fftSize), int(hopSize), float(sampleRate)
+        self._an = CrossSpectral(self.fftSize, 1)  # unsupportedFFTSize surfaces here
+        self._window_spec = window
+        self._w = None
+
+    def frameCount(self, sampleCount: int) -> int:
+        if sampleCount < self.fftSize:
+            return 0
+        return (int(sampleCount) - self.fftSize) // self.hopSize + 1
+
+    def binFrequency(self, bin_: int) -> float:
+        return bin_ * self.sampleRate / self.fftSize
+
+    def frameTime(self, frameIndex: int) -> float:
+        return frameIndex * self.hopSize / self.sampleRate
+
+    def _run(self, signal, want_db: bool):
+        import torch
+        x = _f32(signal)
+        f, n = self.frameCount(x.size), self.fftSize
+        if f == 0:
+            return None
+        if self._w is None:
+            self._w = _window(self._window_spec, n)
+        dx = torch.from_numpy(x).cuda()
+        spec = torch.empty((f, n), dtype=torch.complex64, device="cuda")
+        pw = torch.empty((f, n), dtype=torch.float32, device="cuda")
+        db = torch.empty((f, n), dtype=torch.float32, device="cuda") if want_db else None
+        self._an.stft_device(dx, x.size, self.hopSize, self._w, spec, pw, db)
+        torch.cuda.synchronize()
+        return spec.cpu().numpy(), pw.cpu().numpy(), None if db is None else db.cpu().numpy()
+
+    def analyze(self, signal) -> list:
+        r = self._run(signal, False)
+        if r is None:
+            return []
+        spec, pw, _ = r
+        return [STFTFrame(spec[i], pw[i]) for i in range(spec.shape[0])]
+
+    def spectrogram(self, signal) -> np.ndarray:
+        """[time][frequency] dB magnitudes (STFT.swift:82-85)."""
+        r = self._run(signal, True)
+        return np.zeros((0, self.fftSize), np.float32) if r is None else r[2]
+
+    def analyze_device(self, signals, samples: int, spectra, power=None, power_db=None,
+                       db_alpha: float = 20.0, stream=None) -> None:
+        """Device form over one channel: spectra (frames, N) complex64."""
+        if self._w is None:
+            self._w = _window(self._window_spec, self.fftSize)
+        self._an.stft_device(signals, samples, self.hopSize, self._w, spectra, power, power_db,
+                             db_alpha, stream)
+
+    def close(self) -> None:
+        self._an.close()
+
+
+class AnalysisResult:
+    """AnalysisResult (Sources/MetalFFT/DSP/FFTAnalyzer.swift:5-61):
+    complex spectrum with magnitudes, magnitudesDB, phase, binFrequency and
+    rms = sqrt(sum |X|^2) / N.  Pitch and noise heuristics are not part of
+    this package."""
+
+    def __init__(self, complex_, magnitudes, sampleRate: float, fftSize: int):
+        self.complex, self.magnitudes = complex_, magnitudes
+        self.sampleRate, self.fftSize = sampleRate, fftSize
+
+    @property
+    def magnitudesDB(self) -> np.ndarray:
+        return to_decibels(self.magnitudes)
+
+    @property
+    def phase(self) -> np.ndarray:
+        return np.arctan2(self.complex.imag, self.complex.real).astype(np.float32)
+
+    def binFrequency(self, bin_: int) -> float:
+        return bin_ * self.sampleRate / self.fftSize
+
+    @property
+    def rms(self) -> float:
+        return float(np.sqrt(np.float32(self.magnitudes.sum(dtype=np.float32)))) / self.fftSize
+
+
+class FFTAnalyzer:
+    """FFTAnalyzer (Sources/MetalFFT/DSP/FFTAnalyzer.swift:66-116): window ->
+    FFT of one frame, the one-frame case of the batched STFT launch."""
+
+    def __init__(self, size: int, sampleRate: float, window="hann"):
+        self.size, self.sampleRate = int(size), float(sampleRate)
+        self._stft = STFT(self.size, 1, window, self.sampleRate)
+
+    def analyze(self, samples, offset: int = 0) -> AnalysisResult:
+        x = _f32(samples)
+        if offset == 0 and x.size != self.size:
+            raise FFTError(6, "samples.count must equal size", expected=self.size, got=x.size)
+        if offset < 0 or offset + self.size > x.size:
+            raise FFTError(10, "offset + size must be <= samples.count")
+        spec, pw, _ = self._stft._run(x[offset:offset + self.size], False)
+        return AnalysisResult(spec[0], pw[0], self.sampleRate, self.size)
+
+    def close(self) -> None:
+        self._stft.close()
+
+
+class PSD:
+    """PSD (Sources/MetalFFT/DSP/PSD.swift, CrossSpectral.swift): Welch power
+    spectral density, magnitude-squared coherence and cross-spectral
+    analysis, each over all frames in batched launches."""
+
+    @staticmethod
+    def welch(signal, fftSize: int, hopSize: int, sampleRate: float, window="hann") -> np.ndarray:
+        """PSD.welch (PSD.swift:18-59): mean |X|^2 / (sum w^2 * sampleRate)."""
+        import torch
+        if int(hopSize) < 1:
+            raise FFTError(10, "hopSize must be >= 1")
+        x = _f32(signal)
+        an = CrossSpectral(int(fftSize), 1)
+        try:
+            n = an.fftSize
+            w = _window(window, n)
+            out = torch.empty(n, dtype=torch.float32, device="cuda")
+            dx = torch.from_numpy(x).cuda() if x.size else None
+            an.welch_device(dx, x.size, int(hopSize), w, float(sampleRate), out)
+            torch.cuda.synchronize()
+            return out.cpu().numpy()
+        finally:
+            an.close()
+
+    @staticmethod
+    def coherence(a, b, fftSize: int, hopSize: int, sampleRate: float, window="hann") -> np.ndarray:
+        """PSD.coherence (PSD.swift:72-134): |S_ab|^2 / (S_aa S_bb) per bin,
+        frames over min(len a, len b), 0 where S_aa S_bb is 0."""
+        if int(hopSize) < 1:
+            raise FFTError(10, "hopSize must be >= 1")
+        a, b = _f32(a), _f32(b)
+        m = min(a.size, b.size)
+        r = cross_spectral([a[:m], b[:m]], fftSize, hopSize, sampleRate, window)
+        return r.welch_msc[0]
+
+    @staticmethod
+    def crossSpectral(channels, fftSize: int, hopSize: int, sampleRate: float = 0.0,
+                      window="hann") -> CrossSpectralResult:
+        return cross_spectral(channels, fftSize, hopSize, sampleRate, window)
+
+
 __all__ = ["Convolver", "FusedConvolver", "Correlator", "Precision", "FFTError", "CrossSpectral",
-           "CrossSpectralResult", "cross_spectral", "window_coefficients"]
+           "CrossSpectralResult", "cross_spectral", "window_coefficients", "STFT", "STFTFrame",
+           "FFTAnalyzer", "AnalysisResult", "PSD", "to_decibels"]

@@ -280,10 +280,72 @@ def f64_cross_spectral(channels, fft_size, hop, window=None):
     power, cross, coh = power / f, cross / f, coh / f
     msc = np.zeros_like(coh)
     for q, (i, j) in enumerate(pairs):
-        msc[q] = np.abs(cross[q]) ** 2 / np.maximum(power[i] * power[j], 1e-30)
+        msc[q] = _msc(cross[q], power[i], power[j])
     if not starts:
         msc[:] = 0
     return power, cross, coh, msc
+
+
+def _msc(sxy, sxx, syy):
+    """|S_xy|^2 / (S_xx S_yy), 0 where the denominator is not > 0
+    (PSD.coherence, Sources/MetalFFT/DSP/PSD.swift:127-133)."""
+    den = sxx * syy
+    out = np.zeros(np.shape(den))
+    ok = den > 0
+    out[ok] = np.abs(sxy[ok]) ** 2 / den[ok]
+    return out
+
+
+# ------------------------------------------------ STFT / Welch (FP64)
+def f64_stft(signal, fft_size, hop, window=None):
+    """STFT.analyze (Sources/MetalFFT/DSP/STFT.swift:71-99): frames at
+    pos = 0, hop, ... while pos + N <= samples, each windowed and
+    transformed.  Returns complex spectra [frames][N] (empty when there is
+    no frame)."""
+    x = np.asarray(signal, np.float64).reshape(-1)
+    n = int(fft_size)
+    w = np.ones(n) if window is None else np.asarray(window, np.float64)
+    starts = list(range(0, x.size - n + 1, hop)) if x.size >= n else []
+    if not starts:
+        return np.zeros((0, n), np.complex128)
+    return np.fft.fft(np.stack([x[p:p + n] * w for p in starts]), axis=1)
+
+
+def to_decibels(power, alpha=20.0, floor_db=-120.0):
+    """Spectrum.toDecibels (Sources/MetalFFT/DSP/Spectrum.swift:47-55):
+    alpha * log10(power) (vDSP_vdbcon: alpha 20 for its flag 1, 10 for flag
+    0), clipped below at floor_db; zero power gives the floor."""
+    p = np.asarray(power, np.float64)
+    with np.errstate(divide="ignore"):
+        db = alpha * np.log10(p)
+    return np.maximum(db, floor_db)
+
+
+def f64_welch(signal, fft_size, hop, sample_rate, window=None):
+    """PSD.welch (Sources/MetalFFT/DSP/PSD.swift:18-59): sum over frames of
+    |X|^2 / (frames * sum(w^2) * sample_rate); zeros when there is no
+    frame."""
+    X = f64_stft(signal, fft_size, hop, window)
+    n = int(fft_size)
+    if X.shape[0] == 0:
+        return np.zeros(n)
+    w = np.ones(n) if window is None else np.asarray(window, np.float64)
+    return (np.abs(X) ** 2).sum(axis=0) / (X.shape[0] * float((w * w).sum()) * sample_rate)
+
+
+def f64_coherence(a, b, fft_size, hop, window=None):
+    """PSD.coherence (Sources/MetalFFT/DSP/PSD.swift:72-134): frames over
+    min(len a, len b); |sum A conj B|^2 / (sum |A|^2 sum |B|^2), 0 where the
+    denominator is not > 0 or there is no frame."""
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    m = min(a.size, b.size)
+    A = f64_stft(a[:m], fft_size, hop, window)
+    B = f64_stft(b[:m], fft_size, hop, window)
+    if A.shape[0] == 0:
+        return np.zeros(int(fft_size))
+    return _msc((A * np.conj(B)).sum(axis=0), (np.abs(A) ** 2).sum(axis=0),
+                (np.abs(B) ** 2).sum(axis=0))
 
 
 # ---------------------------------------------------------------- metrics

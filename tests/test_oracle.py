@@ -302,3 +302,79 @@ def test_cross_spectral_frames_and_independence(oracle):
     assert 0.2 / frames < ms.mean() < 3.0 / frames
     none = oracle.f64_cross_spectral(x[:, :n - 1], n, hop)
     assert all(not np.any(a) for a in none)
+
+
+# ------------------------------------------------------ STFT / Welch / MSC
+def test_stft_frames_parseval_and_brute_force(oracle):
+    """Frame count (pos + N <= samples); each frame is the brute-force DFT of
+    the windowed slice; Parseval per frame; frame f+1 of x is frame f of
+    x shifted by one hop."""
+    rng = np.random.default_rng(11)
+    n, hop = 64, 24
+    x = rng.standard_normal(5 * n + 7)
+    w = np.hanning(n + 1)[:n]
+    X = oracle.f64_stft(x, n, hop, w)
+    assert X.shape == ((x.size - n) // hop + 1, n)
+    for f in (0, X.shape[0] - 1):
+        seg = x[f * hop:f * hop + n] * w
+        assert oracle.rel_l2(X[f], oracle.dft_brute_f64(seg.astype(np.complex128))) < 1e-13
+        assert abs((np.abs(X[f]) ** 2).sum() - n * (seg ** 2).sum()) < 1e-9 * n * (seg ** 2).sum()
+    assert np.allclose(oracle.f64_stft(x[hop:], n, hop, w)[0], X[1])
+    assert oracle.f64_stft(x[:n - 1], n, hop, w).shape == (0, n)
+
+
+def test_to_decibels_known_values(oracle):
+    p = np.array([0.0, 1.0, 100.0, 1e-20, 1e-3])
+    assert np.array_equal(oracle.to_decibels(p, 20.0), [-120.0, 0.0, 40.0, -120.0, -60.0])
+    assert np.allclose(oracle.to_decibels(p, 10.0), [-120.0, 0.0, 20.0, -120.0, -30.0])
+
+
+def test_welch_constant_signal_closed_form(oracle):
+    """A constant a through the periodic Hann window (sum w = N/2, sum w^2 =
+    3N/8): X[0] = aN/2, X[+-1] = -aN/4, all else 0, the same in every frame,
+    so psd[0] = 2 a^2 N / (3 fs), psd[1] = psd[N-1] = a^2 N / (6 fs)."""
+    from metalfft_amd.dsp import window_coefficients
+    n, hop, a, fs = 256, 96, 0.6, 8000.0
+    w = window_coefficients("hann", n).astype(np.float64)
+    assert abs(w.sum() - n / 2) < 1e-4 and abs((w * w).sum() - 3 * n / 8) < 1e-4
+    psd = oracle.f64_welch(np.full(7 * n, a), n, hop, fs, w)
+    want = np.zeros(n)
+    want[0] = 2 * a * a * n / (3 * fs)
+    want[1] = want[-1] = a * a * n / (6 * fs)
+    assert np.allclose(psd, want, rtol=1e-5, atol=1e-12)
+    assert not np.any(oracle.f64_welch(np.ones(n - 1), n, hop, fs, w))
+
+
+def test_coherence_closed_forms(oracle):
+    """b = 3a: coherence 1 wherever a has power; b = 0: 0 everywhere (the
+    denominator guard); independent noise ~ 1/frames; the longer signal is
+    cut to the shorter one's length."""
+    rng = np.random.default_rng(12)
+    n, hop = 128, 64
+    a = rng.standard_normal(40 * n)
+    c = oracle.f64_coherence(a, 3 * a, n, hop)
+    assert np.allclose(c, 1.0)
+    assert not np.any(oracle.f64_coherence(a, np.zeros_like(a), n, hop))
+    b = rng.standard_normal(a.size + 500)
+    c = oracle.f64_coherence(a, b, n, hop)
+    frames = (a.size - n) // hop + 1
+    assert 0.2 / frames < c.mean() < 3.0 / frames
+    assert np.array_equal(c, oracle.f64_coherence(a, b[:a.size], n, hop))
+    assert np.allclose(oracle.f64_cross_spectral([a, b[:a.size]], n, hop)[3][0], c)
+
+
+def test_window_definitions():
+    """Periodic cosine windows: sums over a period are closed forms (a0 N);
+    Kaiser with beta 0 is rectangular, is symmetric, and its I0 series
+    agrees with scipy's Bessel I0."""
+    import scipy.special
+    from metalfft_amd.dsp import _bessel_i0, window_coefficients
+    n = 256
+    for kind, a0 in (("hann", 0.5), ("hamming", 0.54), ("blackman", 0.42),
+                     ("flatTop", 0.21557895), ("rectangular", 1.0)):
+        assert abs(window_coefficients(kind, n).astype(np.float64).sum() - a0 * n) < 1e-4 * n
+    assert np.allclose(window_coefficients(("kaiser", 0.0), n), 1.0)
+    k = window_coefficients(("kaiser", 8.6), n)
+    assert np.allclose(k, k[::-1]) and k.max() <= 1.0 + 1e-7
+    for x in (0.0, 0.5, 3.0, 8.6, 14.0):
+        assert abs(_bessel_i0(x) - scipy.special.i0(x)) < 1e-11 * scipy.special.i0(x)
