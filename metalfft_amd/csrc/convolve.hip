@@ -731,24 +731,30 @@ mifft_status convolver_backward(mifft_convolver* cv, const float* x, int64_t S, 
                                 nbx, npx, (int)ppw, st),
                        "fused convolve adjoint")) != MIFFT_OK)
         return s;
-    // grad_h = column sums of the slot partials: 64-row levels, ping-pong
-    // between two scratch areas after `part`, the last level into gh
+    // grad_h = column sums of the slot partials, ping-pong between two
+    // scratch areas after `part`
     const int K = (int)cv->K;
-    const float* src = part;
-    float* scratch[2] = {part + slots * (int64_t)K, part + (slots + level1) * (int64_t)K};
-    int64_t rows = slots;
+    return check_hip(sum_rows(part, slots, K, part + slots * (int64_t)K, part + (slots + level1) * (int64_t)K,
+                              gh, st),
+                     "tap gradient reduction");
+}
+
+hipError_t sum_rows(const float* in, int64_t rows, int cols, float* s0, float* s1, float* out,
+                    hipStream_t st) {
+    const float* src = in;
+    float* scratch[2] = {s0, s1};
     for (int lvl = 0; rows > 1; ++lvl) {
         const int64_t next = (rows + 63) / 64;
-        float* dst = next == 1 ? gh : scratch[lvl & 1];
-        hipLaunchKernelGGL(k_sum_rows64, dim3((unsigned)((K + 63) / 64), (unsigned)next), dim3(256), 0,
-                           st, src, dst, rows, K);
+        float* dst = next == 1 ? out : scratch[lvl & 1];
+        hipLaunchKernelGGL(k_sum_rows64, dim3((unsigned)((cols + 63) / 64), (unsigned)next), dim3(256), 0, st,
+                           src, dst, rows, cols);
         src = dst;
         rows = next;
     }
-    if (slots == 1)  // one slot: its partial is the gradient
-        hipLaunchKernelGGL(k_sum_rows64, dim3((unsigned)((K + 63) / 64), 1), dim3(256), 0, st, part, gh,
-                           (int64_t)1, K);
-    return check_hip(hipGetLastError(), "tap gradient reduction");
+    if (src == in)  // one row: copied as it is
+        hipLaunchKernelGGL(k_sum_rows64, dim3((unsigned)((cols + 63) / 64), 1), dim3(256), 0, st, in, out,
+                           (int64_t)1, cols);
+    return hipGetLastError();
 }
 
 // Fill kernel rather than hipMemsetAsync, so callers may capture it in a graph.

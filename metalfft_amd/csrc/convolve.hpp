@@ -28,6 +28,11 @@ mifft_status convolver_backward(mifft_convolver* cv, const float* x, int64_t S, 
 // (each truncated or zero-padded to N = plan size).
 mifft_status correlate_device(const mifft_plan* plan, const float* a, int64_t na, const float* b,
                               int64_t nb, float* r, int64_t count, hipStream_t st);
+// out[c] = sum over r < rows of in[r * cols + c], in 64-row levels added in a
+// fixed order (deterministic); s0, s1: scratch of ceil(rows / 64) * cols
+// floats each (only used when rows > 64).
+hipError_t sum_rows(const float* in, int64_t rows, int cols, float* s0, float* s1, float* out,
+                    hipStream_t st);
 // p[0:n] <- 0 with a fill kernel.
 mifft_status zero_floats(float* p, int64_t n, hipStream_t st);
 }  // namespace mifft

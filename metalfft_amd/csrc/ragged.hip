@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "kernels.hpp"
 #include "ragged.hpp"
 #include "stockham_core.hpp"
 
@@ -32,57 +33,77 @@ constexpr int kTileElems = 16384;
 constexpr int kTileThreads = kTileElems / 16;
 constexpr int kTileLds = kTileElems + kTileElems / 16;  // LdsPad::len(N) * (16384 / N) for all N
 
+// Twiddle tables of every length back to back: the plan table of N = 2^l
+// (W_N^k, k < N, then the packed bases rows of PassTable<N, 16, 0>) starts at
+// table_off(l).
+constexpr int table_len(int l) { return l == 6 ? 64 + PassTable<64, 16, 0>::size
+                                      : l == 7 ? 128 + PassTable<128, 16, 0>::size
+                                      : l == 8 ? 256 + PassTable<256, 16, 0>::size
+                                      : l == 9 ? 512 + PassTable<512, 16, 0>::size
+                                      : l == 10 ? 1024 + PassTable<1024, 16, 0>::size
+                                      : l == 11 ? 2048 + PassTable<2048, 16, 0>::size
+                                      : l == 12 ? 4096 + PassTable<4096, 16, 0>::size
+                                      : l == 13 ? 8192 + PassTable<8192, 16, 0>::size
+                                      : 16384 + PassTable<16384, 16, 0>::size; }
+constexpr int table_off(int l) { return l <= 6 ? 0 : table_off(l - 1) + table_len(l - 1); }
+constexpr int kTablesLen = table_off(15);
+
+// The transform's data loads are issued first, then its packed twiddle
+// bases rows (one 32-byte row per butterfly and pass), so no pass waits on a
+// table read.
 template <int N, bool INV>
-__device__ __noinline__ void tile_body(const float2* in, float2* out, const float2* tw,
+__device__ __forceinline__ void tile_body(const float2* in, float2* out, const float2* tw,
                                           const int64_t* offs, int count, float2* lds) {
     constexpr int E = 16;
     constexpr int T = N / E;
-    const int local = threadIdx.x / T;
-    const int t = threadIdx.x % T;
+    const int tid = fresh((int)threadIdx.x);  // index arithmetic redone per tile, not hoisted
+    const int local = tid / T;
+    const int t = tid % T;
     const bool active = local < count;
     const int64_t off = active ? offs[local] : 0;
     float2* buf = lds + local * LdsPad::len(N);
     float2 v[E];
     load_column<N, E, INV, false>(v, in + off, t, active);
-    stockham_regs<N, E, LdsPad, 0, true>(v, buf, t, tw);
+    PreBases<N, E> pb;
+    pb.load_packed(tw, t);
+    stockham_regs<N, E, LdsPad, 0, true, true, false, 2>(v, buf, t, tw, pb.w);
     if (active) store_column<N, E, INV, false>(v, out + off, t);
 }
 
-// tw_all holds W_N^k for every length back to back: table N starts at N - 64.
-template <bool INV>
-__global__ void __launch_bounds__(kTileThreads)
-k_ragged(const float2* in, float2* out, const float2* __restrict__ tw_all,
-         const RaggedTile* __restrict__ tiles, const int64_t* __restrict__ offs, int tile_begin,
-         int tile_end) {
-    __shared__ float2 lds[kTileLds];
-    for (int ti = tile_begin + (int)blockIdx.x; ti < tile_end; ti += (int)gridDim.x) {
+// Workgroup b takes tiles tile_begin + b, + gridDim.x, ... of the sorted
+// list; the tiles of one length form a contiguous range, so that sequence is
+// walked as one loop per length (largest first), each with its own
+// transform code.
+template <int L, bool INV>
+__device__ __forceinline__ void length_loop(const float2* in, float2* out, const float2* tw_all,
+                                            const RaggedTile* tiles, const int64_t* offs,
+                                            const RaggedRanges& rr, int tile_begin, int tile_end,
+                                            float2* lds) {
+    const int b = rr.lo[L - 6] > tile_begin ? rr.lo[L - 6] : tile_begin;
+    const int e = rr.hi[L - 6] < tile_end ? rr.hi[L - 6] : tile_end;
+    const int stride = (int)gridDim.x;
+    const int r = tile_begin + (int)blockIdx.x;
+    for (int ti = b + (((r - b) % stride) + stride) % stride; ti < e; ti += stride) {
         const RaggedTile d = tiles[ti];
-        const int64_t* o = offs + d.first;
-        switch (d.log2n) {  // uniform across the workgroup
-            case 6: tile_body<64, INV>(in, out, tw_all + 0, o, d.count, lds); break;
-            case 7: tile_body<128, INV>(in, out, tw_all + 64, o, d.count, lds); break;
-            case 8: tile_body<256, INV>(in, out, tw_all + 192, o, d.count, lds); break;
-            case 9: tile_body<512, INV>(in, out, tw_all + 448, o, d.count, lds); break;
-            case 10: tile_body<1024, INV>(in, out, tw_all + 960, o, d.count, lds); break;
-            case 11: tile_body<2048, INV>(in, out, tw_all + 1984, o, d.count, lds); break;
-            case 12: tile_body<4096, INV>(in, out, tw_all + 4032, o, d.count, lds); break;
-            case 13: tile_body<8192, INV>(in, out, tw_all + 8128, o, d.count, lds); break;
-            case 14: tile_body<16384, INV>(in, out, tw_all + 16320, o, d.count, lds); break;
-            default: break;
-        }
+        tile_body<(1 << L), INV>(in, out, tw_all + table_off(L), offs + d.first, d.count, lds);
     }
 }
 
-__global__ void k_twiddles_all(float2* __restrict__ tw) {
-    // table for N = 2^l occupies [N - 64, 2N - 64)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 32704) return;
-    int n = 64;
-    while (i >= 2 * n - 64) n *= 2;
-    const int k = i - (n - 64);
-    double s, c;
-    sincospi(-2.0 * (double)k / (double)n, &s, &c);
-    tw[i] = make_float2((float)c, (float)s);
+template <bool INV>
+__global__ void __launch_bounds__(kTileThreads)
+k_ragged(const float2* in, float2* out, const float2* __restrict__ tw_all,
+         const RaggedTile* __restrict__ tiles, const int64_t* __restrict__ offs, RaggedRanges rr,
+         int tile_begin, int tile_end) {
+    __shared__ float2 lds[kTileLds];
+    length_loop<14, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<13, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<12, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<11, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<10, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<9, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<8, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<7, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    length_loop<6, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
 }
 
 }  // namespace
@@ -98,23 +119,28 @@ int ragged_cus() {
     return cus;
 }
 
+int ragged_twiddles_len() { return kTablesLen; }
+
 hipError_t launch_ragged_twiddles(float2* tw_all, hipStream_t st) {
-    hipLaunchKernelGGL(k_twiddles_all, dim3((32704 + 255) / 256), dim3(256), 0, st, tw_all);
-    return hipGetLastError();
+    for (int l = 6; l <= 14; ++l) {
+        const hipError_t e = launch_twiddles(tw_all + table_off(l), 1 << l, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_ragged(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tiles,
-                         const int64_t* offs, int tile_begin, int tile_end, bool inverse,
-                         hipStream_t st) {
+                         const int64_t* offs, const RaggedRanges& rr, int tile_begin, int tile_end,
+                         bool inverse, hipStream_t st) {
     const int ntiles = tile_end - tile_begin;
     if (ntiles <= 0) return hipSuccess;
     const int grid = std::min(ntiles, ragged_cus());
     if (inverse)
         hipLaunchKernelGGL(k_ragged<true>, dim3(grid), dim3(kTileThreads), 0, st, in, out, tw_all,
-                           tiles, offs, tile_begin, tile_end);
+                           tiles, offs, rr, tile_begin, tile_end);
     else
         hipLaunchKernelGGL(k_ragged<false>, dim3(grid), dim3(kTileThreads), 0, st, in, out, tw_all,
-                           tiles, offs, tile_begin, tile_end);
+                           tiles, offs, rr, tile_begin, tile_end);
     return hipGetLastError();
 }
 
@@ -130,10 +156,10 @@ mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool 
     const int64_t* offs = static_cast<const int64_t*>(rp->d_offsets.ptr);
     const float2* tw = static_cast<const float2*>(rp->d_twiddles.ptr);
     if ((mode & MIFFT_RAGGED_PER_SIZE) == 0)
-        return check_hip(launch_ragged(in, out, tw, tiles, offs, 0, rp->ntiles, inverse, st),
+        return check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, 0, rp->ntiles, inverse, st),
                          "ragged fft");
     for (size_t s = 0; s + 1 < rp->size_tile_begin.size(); ++s) {
-        mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->size_tile_begin[s],
+        mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, rp->size_tile_begin[s],
                                                  rp->size_tile_begin[s + 1], inverse, st),
                                    "ragged fft (per size)");
         if (r != MIFFT_OK) return r;
@@ -209,6 +235,11 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
     rp->extent = extent;
     rp->ntiles = (int)tiles.size();
     rp->size_tile_begin = begins;
+    for (size_t q = 0; q + 1 < begins.size(); ++q) {
+        const int l = tiles[(size_t)begins[q]].log2n;
+        rp->ranges.lo[l - 6] = begins[q];
+        rp->ranges.hi[l - 6] = begins[q + 1];
+    }
     for (int64_t a = 0; a < count; ++a) rp->elements += lengths[a];
     if (hipStreamCreateWithFlags(&rp->stream, hipStreamNonBlocking) != hipSuccess) {
         mifft_ragged_plan_destroy(rp);
@@ -216,7 +247,7 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
     }
     if ((s = rp->d_tiles.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tiles.size()))) != MIFFT_OK ||
         (s = rp->d_offsets.reserve(sizeof(int64_t) * std::max<int64_t>(1, count))) != MIFFT_OK ||
-        (s = rp->d_twiddles.reserve(sizeof(float2) * 32704)) != MIFFT_OK) {
+        (s = rp->d_twiddles.reserve(sizeof(float2) * ragged_twiddles_len())) != MIFFT_OK) {
         mifft_ragged_plan_destroy(rp);
         return s;
     }

@@ -12,6 +12,12 @@ struct RaggedTile {
     int32_t count;
     int64_t first;
 };
+// Tiles of length 2^(6 + i) are [lo[i], hi[i]) of the sorted tile list
+// (empty: lo == hi == 0).
+struct RaggedRanges {
+    int32_t lo[9];
+    int32_t hi[9];
+};
 }  // namespace mifft
 
 struct mifft_ragged_plan {
@@ -20,6 +26,7 @@ struct mifft_ragged_plan {
     int64_t extent = 0;    // max(offset + length): buffer length needed
     int ntiles = 0;
     std::vector<int> size_tile_begin;  // tile range per length, largest length first
+    mifft::RaggedRanges ranges = {};   // the same ranges indexed by log2 length
     mifft::DeviceBuffer d_tiles, d_offsets, d_twiddles;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;

@@ -102,6 +102,81 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
     }
 }
 
+// Welch power without a spectra pass: workgroup (c, j) of the wpc per
+// channel transforms frame groups j*iters .. j*iters + iters - 1 (TPW frames
+// each) of channel c and sums |X[k]|^2, k <= N/2, over them in registers;
+// the TPW slots are then added through LDS in a fixed order and written as
+// partial[j * cols + c * (N/2 + 1) + k].
+template <int N, bool WIN>
+__global__ void __launch_bounds__(RCfg<N / 2>::threads)
+k_real_power(const float* __restrict__ sig, int64_t samples, int64_t hop, int64_t frames, int64_t wpc,
+             int64_t iters, const float* __restrict__ window, float* __restrict__ partial, int64_t cols,
+             const float2* __restrict__ tab, const float2* __restrict__ roots) {
+    constexpr int M = N / 2;
+    using C = RCfg<M>;
+    constexpr int E = C::E, T = M / E;
+    __shared__ float2 lds[C::TPW * LdsPad::len(M)];
+    const int local = threadIdx.x / T;
+    const int t = threadIdx.x % T;
+    float2* buf = lds + local * LdsPad::len(M);
+    const int64_t c = blockIdx.x / wpc, j = blockIdx.x % wpc;
+    const float* base = sig + c * samples;
+    float acc[E], acc_m = 0.f;
+#pragma unroll
+    for (int m = 0; m < E; ++m) acc[m] = 0.f;
+    RegTwiddles<M, E> rt;
+    rt.load(tab, t);
+    for (int64_t it = 0; it < iters; ++it) {
+        const int tl = fresh(t);  // per-iteration index arithmetic (not hoisted)
+        const int64_t f = (j * iters + it) * C::TPW + local;
+        const bool act = f < frames;
+        const float* p = base + (act ? f : 0) * hop;
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int q = 2 * (tl + T * m);
+            float x0 = p[q], x1 = p[q + 1];
+            if constexpr (WIN) {
+                x0 *= window[q];
+                x1 *= window[q + 1];
+            }
+            v[m] = make_float2(x0, x1);
+        }
+        rt.template fft<true>(v, buf, tl, tab);
+        lds_barrier();
+#pragma unroll
+        for (int m = 0; m < E; ++m) buf[LdsPad::idx(tl + T * m)] = v[m];
+        lds_barrier();
+        const float keep = act ? 1.f : 0.f;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int k = tl + T * m;
+            const float2 z = v[m];
+            const float2 zc = conjf2(buf[LdsPad::idx((M - k) & (M - 1))]);
+            const float2 e = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+            const float2 o = mul_mi(make_float2(0.5f * (z.x - zc.x), 0.5f * (z.y - zc.y)));
+            const float2 lo = cadd(e, cmul(roots[k], o));
+            acc[m] += keep * (lo.x * lo.x + lo.y * lo.y);
+            if (m == 0) {  // bin M from k = 0: E[0] - O[0]
+                const float hm = e.x - o.x;
+                acc_m += keep * (hm * hm);
+            }
+        }
+    }
+    lds_barrier();
+    float* red = reinterpret_cast<float*>(lds);  // TPW x (M + 1) floats
+#pragma unroll
+    for (int m = 0; m < E; ++m) red[local * (M + 1) + t + T * m] = acc[m];
+    if (t == 0) red[local * (M + 1) + M] = acc_m;
+    lds_barrier();
+    for (int i = threadIdx.x; i <= M; i += C::threads) {
+        float s = 0.f;
+#pragma unroll
+        for (int l = 0; l < C::TPW; ++l) s += red[l * (M + 1) + i];
+        partial[j * cols + c * (M + 1) + i] = s;
+    }
+}
+
 template <int N>
 __global__ void __launch_bounds__(RCfg<N / 2>::threads)
 k_real_inverse(const float2* __restrict__ y, int64_t in_stride, float* __restrict__ x, int64_t batch,
@@ -156,6 +231,24 @@ hipError_t frames_n(const RealFrames& a, hipStream_t st) {
 }
 
 template <int N>
+hipError_t power_n(const RealPower& a, hipStream_t st) {
+    constexpr int TPW = RCfg<N / 2>::TPW;
+    const int64_t blocks = a.channels * a.wpc;
+    if (blocks > 0x7fffffffLL || a.iters * a.wpc * TPW < a.frames) return hipErrorInvalidValue;
+    const float2* roots = a.tab + real_roots_offset(N);
+    const int64_t cols = a.channels * (N / 2 + 1);
+    if (a.window)
+        hipLaunchKernelGGL((k_real_power<N, true>), dim3((unsigned)blocks), dim3(RCfg<N / 2>::threads), 0, st,
+                           a.sig, a.samples, a.hop, a.frames, a.wpc, a.iters, a.window, a.partial, cols, a.tab,
+                           roots);
+    else
+        hipLaunchKernelGGL((k_real_power<N, false>), dim3((unsigned)blocks), dim3(RCfg<N / 2>::threads), 0,
+                           st, a.sig, a.samples, a.hop, a.frames, a.wpc, a.iters, a.window, a.partial, cols,
+                           a.tab, roots);
+    return hipGetLastError();
+}
+
+template <int N>
 hipError_t inverse_n(const float2* y, int64_t in_stride, float* x, int64_t batch, const float2* tab,
                      hipStream_t st) {
     using C = RCfg<N / 2>;
@@ -186,6 +279,23 @@ hipError_t real_frames_forward(int n, const RealFrames& a, hipStream_t st) {
     case NN: return frames_n<NN>(a, st);
     switch (n) { MIFFT_REAL_SIZES(MIFFT_RF) }
 #undef MIFFT_RF
+    return hipErrorInvalidValue;
+}
+
+int real_frames_per_group(int n) {
+#define MIFFT_RT(NN) \
+    case NN: return RCfg<NN / 2>::TPW;
+    switch (n) { MIFFT_REAL_SIZES(MIFFT_RT) }
+#undef MIFFT_RT
+    return 0;
+}
+
+hipError_t real_power(int n, const RealPower& a, hipStream_t st) {
+    if (a.frames <= 0) return hipSuccess;
+#define MIFFT_RP(NN) \
+    case NN: return power_n<NN>(a, st);
+    switch (n) { MIFFT_REAL_SIZES(MIFFT_RP) }
+#undef MIFFT_RP
     return hipErrorInvalidValue;
 }
 

@@ -33,6 +33,22 @@ struct RealFrames {
 };
 hipError_t real_frames_forward(int n, const RealFrames& a, hipStream_t st);
 
+// Welch power sums without a spectra pass: frames of `channels` signals as
+// in RealFrames (per channel, `frames` frames), workgroup (c, j), j < wpc,
+// sums |X[k]|^2, k <= N/2, over frames [j * iters * G, (j + 1) * iters * G)
+// (G = real_frames_per_group(n) frames per transform group) and writes
+// partial[j * channels * (N/2 + 1) + c * (N/2 + 1) + k].  Needs
+// wpc * iters * G >= frames.
+struct RealPower {
+    const float* sig;
+    int64_t samples, hop, frames, channels, wpc, iters;
+    const float* window;
+    float* partial;
+    const float2* tab;
+};
+int real_frames_per_group(int n);
+hipError_t real_power(int n, const RealPower& a, hipStream_t st);
+
 // Inverse of half spectra: rows y[b * in_stride + k], k <= N/2, of Hermitian
 // spectra -> x[b * N + j] = (1/N) sum_k X[k] W^{-jk} (real).
 hipError_t real_inverse(int n, const float2* y, int64_t in_stride, float* x, int64_t batch,

@@ -31,6 +31,7 @@
 
 #include <string>
 
+#include "convolve.hpp"
 #include "realfft.hpp"
 #include "runtime.hpp"
 #include "stockham_core.hpp"
@@ -158,12 +159,15 @@ __global__ void __launch_bounds__(256) k_sumsq(const float* __restrict__ w, int6
     if (threadIdx.x == 0) out[0] = part[0];
 }
 
-// psd[i] = power[i] / (window sum of squares * sample rate) (PSD.swift:55-58;
-// power is already divided by the frame count).
-__global__ void k_psd_scale(float* __restrict__ psd, const float* __restrict__ sumsq, int64_t n,
-                            float inv_rate) {
+// psd[c][k] = sum of |X[k]|^2 over frames / (frames * window sum of squares
+// * sample rate) (PSD.swift:55-58), k < N, from sums[c][k'] over bins
+// k' <= N/2 (bin k > N/2 is bin N - k).
+__global__ void k_psd_finish(const float* __restrict__ sums, const float* __restrict__ sumsq, int64_t n,
+                             int64_t total, float inv_frames_rate, float* __restrict__ psd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) psd[i] = psd[i] * (inv_rate / sumsq[0]);
+    if (i >= total) return;
+    const int64_t c = i / n, k = i % n, nb = n / 2 + 1;
+    psd[i] = sums[c * nb + (k < nb ? k : n - k)] * (inv_frames_rate / sumsq[0]);
 }
 
 mifft_status zero(float* p, int64_t n, hipStream_t st) {
@@ -295,14 +299,27 @@ mifft_status mifft_welch_device(mifft_xspec* x, const float* signals, int64_t sa
     const int64_t frames = mifft_xspec_frames(x, samples, hop);
     if (frames == 0) return zero(psd, c * n, st);  // the reference returns its zeroed accumulator
     if (!signals) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null signals");
+    // workgroups: about 2048 over all channels, each summing `iters` groups
+    const int64_t nb = n / 2 + 1, cols = c * nb;
+    const int64_t grp = real_frames_per_group((int)n);
+    const int64_t groups = (frames + grp - 1) / grp;
+    int64_t wpc = (2048 + c - 1) / c;
+    if (wpc > groups) wpc = groups;
+    const int64_t iters = (groups + wpc - 1) / wpc;
+    const int64_t lvl1 = (wpc + 63) / 64;
     mifft_status s;
-    if ((s = x->aux.reserve(sizeof(float))) != MIFFT_OK) return s;
-    float* sumsq = static_cast<float*>(x->aux.ptr);
-    if ((s = xspec_run(x, signals, samples, hop, frames, window, psd, nullptr, nullptr, 0, st)) != MIFFT_OK)
+    if ((s = x->aux.reserve(sizeof(float) * (size_t)((wpc + 2 * lvl1 + 1) * cols + 1))) != MIFFT_OK) return s;
+    float* part = static_cast<float*>(x->aux.ptr);
+    float* sums = part + (wpc + 2 * lvl1) * cols;
+    float* sumsq = sums + cols;
+    const RealPower a{signals, samples, hop, frames, c, wpc, iters, window, part, x->plan->real_tw};
+    if ((s = check_hip(real_power((int)n, a, st), "Welch power")) != MIFFT_OK ||
+        (s = check_hip(sum_rows(part, wpc, (int)cols, part + wpc * cols, part + (wpc + lvl1) * cols, sums, st),
+                       "Welch reduction")) != MIFFT_OK)
         return s;
     hipLaunchKernelGGL(k_sumsq, dim3(1), dim3(256), 0, st, window, n, sumsq);
-    hipLaunchKernelGGL(k_psd_scale, dim3((unsigned)((c * n + 255) / 256)), dim3(256), 0, st, psd, sumsq,
-                       c * n, (float)(1.0 / sample_rate));
+    hipLaunchKernelGGL(k_psd_finish, dim3((unsigned)((c * n + 255) / 256)), dim3(256), 0, st, sums, sumsq, n,
+                       c * n, (float)(1.0 / ((double)frames * sample_rate)), psd);
     return check_hip(hipGetLastError(), "Welch PSD");
 }
 

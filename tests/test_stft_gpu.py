@@ -94,7 +94,8 @@ def test_fft_analyzer(gpu, oracle):
 
 
 @pytest.mark.parametrize("n,hop,s,fs", [(1024, 512, 44100, 44100.0), (64, 10, 5000, 1000.0),
-                                        (8192, 2048, 100000, 96000.0)])
+                                        (8192, 2048, 100000, 96000.0), (64, 1, 200000, 8000.0),
+                                        (16384, 3000, 16384, 48000.0), (256, 7, 300000, 44100.0)])
 def test_welch_vs_fp64(gpu, oracle, n, hop, s, fs):
     x = rnd(s, n)
     got = gpu.PSD.welch(x, n, hop, fs)
