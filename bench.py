@@ -368,6 +368,37 @@ def extra_xspec(m, torch, c=8, s=1 << 20, n=1024, hop=512):
             "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
+def extra_sizes(m, torch):
+    """Every supported length at 2^24 complex values per launch (128 MiB in,
+    128 MiB out), four rotating sets, the autotuned kernel, graph replay of
+    40 launches: transforms/s and fraction of the HBM peak."""
+    import ctypes
+    from metalfft_amd import _lib
+    L = _lib.load()
+    out = {}
+    for n in m.SUPPORTED_SIZES:
+        batch = (1 << 24) // n
+        f = m.MetalFFT(n)
+        sets = [(torch.randn((batch, n), dtype=torch.complex64, device="cuda"),
+                 torch.empty((batch, n), dtype=torch.complex64, device="cuda")) for _ in range(4)]
+
+        def step(i, on_stream, f=f, sets=sets, batch=batch):
+            src, dst = sets[i % 4]
+            st = ctypes.c_void_p(on_stream if on_stream is not None
+                                 else torch.cuda.current_stream().cuda_stream)
+            _lib.check(L.mifft_execute_device(f._plan, ctypes.c_void_p(src.data_ptr()),
+                                              ctypes.c_void_p(dst.data_ptr()), ctypes.c_int64(batch),
+                                              ctypes.c_int(m.FORWARD), st))
+        ms, _, graph = time_graph(step, 40, 4, torch)
+        del graph
+        gbs = 16 * n * batch / (ms * 1e-3) / 1e9
+        out[str(n)] = {"kernel": f.kernel_name, "us": ms * 1e3, "transforms_per_s": batch / (ms * 1e-3),
+                       "gbs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS}
+        del sets
+        torch.cuda.empty_cache()
+    return out
+
+
 def extra_real(m, torch, n=4096, batch=4096):
     """Real-input transforms, 4096 real signals of 4096 samples: forward to
     all N bins, to bins 0..N/2, and the inverse from bins 0..N/2."""
@@ -503,7 +534,8 @@ def main():
 
     extras = {}
     if not a.no_extras:
-        for key, fn in (("mfma", lambda: extra_mfma(m, torch, O)),
+        for key, fn in (("sizes_128MiB", lambda: extra_sizes(m, torch)),
+                        ("mfma", lambda: extra_mfma(m, torch, O)),
                         ("ragged_10k", lambda: extra_ragged(m, torch, cpu=not a.no_cpu)),
                         ("convolve_512x65536_k1023",
                          lambda: extra_convolve(m, torch, cpu=not a.no_cpu)),
