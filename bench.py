@@ -371,7 +371,7 @@ def extra_xspec(m, torch, c=8, s=1 << 20, n=1024, hop=512):
 def extra_sizes(m, torch):
     """Every supported length at 2^24 complex values per launch (128 MiB in,
     128 MiB out), four rotating sets, the autotuned kernel, graph replay of
-    40 launches: transforms/s and fraction of the HBM peak."""
+    200 launches: transforms/s and fraction of the HBM peak."""
     import ctypes
     from metalfft_amd import _lib
     L = _lib.load()
@@ -389,7 +389,7 @@ def extra_sizes(m, torch):
             _lib.check(L.mifft_execute_device(f._plan, ctypes.c_void_p(src.data_ptr()),
                                               ctypes.c_void_p(dst.data_ptr()), ctypes.c_int64(batch),
                                               ctypes.c_int(m.FORWARD), st))
-        ms, _, graph = time_graph(step, 40, 4, torch)
+        ms, _, graph = time_graph(step, 200, 8, torch)
         del graph
         gbs = 16 * n * batch / (ms * 1e-3) / 1e9
         out[str(n)] = {"kernel": f.kernel_name, "us": ms * 1e3, "transforms_per_s": batch / (ms * 1e-3),

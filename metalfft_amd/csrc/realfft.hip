@@ -40,7 +40,24 @@ __global__ void k_half_roots(float2* __restrict__ r, int n) {
     }
 }
 
-template <int N, bool WIN>
+// Samples j, j + 1 of a frame (times the window): one 8-byte load when the
+// frame start and the window are 8-byte aligned (PAIR), else two.
+template <bool WIN, bool PAIR>
+__device__ __forceinline__ float2 ld_pair(const float* p, const float* w, int j) {
+    float2 x;
+    if constexpr (PAIR) {
+        x = *reinterpret_cast<const float2*>(p + j);
+    } else {
+        x = make_float2(p[j], p[j + 1]);
+    }
+    if constexpr (WIN) {
+        const float2 g = PAIR ? *reinterpret_cast<const float2*>(w + j) : make_float2(w[j], w[j + 1]);
+        x = make_float2(x.x * g.x, x.y * g.y);
+    }
+    return x;
+}
+
+template <int N, bool WIN, bool PAIR>
 __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames a, const float2* roots) {
     constexpr int M = N / 2;
     using C = RCfg<M>;
@@ -55,15 +72,7 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
     const float* p = a.sig + (gc / a.frames) * a.samples + (gc % a.frames) * a.hop;
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-        const int j = 2 * (t + T * m);
-        float x0 = p[j], x1 = p[j + 1];  // whole frames: always in bounds
-        if constexpr (WIN) {
-            x0 *= a.window[j];
-            x1 *= a.window[j + 1];
-        }
-        v[m] = make_float2(x0, x1);
-    }
+    for (int m = 0; m < E; ++m) v[m] = ld_pair<WIN, PAIR>(p, a.window, 2 * (t + T * m));  // whole frames
     RegTwiddles<M, E> rt;
     rt.load(a.tab, t);
     rt.template fft<false>(v, buf, t, a.tab);
@@ -84,19 +93,20 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
         const float2 lo = cadd(e, wo), hi = csub(e, wo);
         const bool put_hi = full || k == 0;
         const int khi = full ? k + M : M;
-        if (act) {
-            out[k] = lo;
-            if (put_hi) out[khi] = hi;
+        if (act) {  // streamed outputs: non-temporal stores
+            gstore<true>(out + k, lo);
+            if (put_hi) gstore<true>(out + khi, hi);
         }
         if (a.pw || a.db) {
             const float ql = lo.x * lo.x + lo.y * lo.y, qh = hi.x * hi.x + hi.y * hi.y;
             if (a.pw && act) {
-                a.pw[gc * a.bins + k] = ql;
-                if (put_hi) a.pw[gc * a.bins + khi] = qh;
+                __builtin_nontemporal_store(ql, a.pw + gc * a.bins + k);
+                if (put_hi) __builtin_nontemporal_store(qh, a.pw + gc * a.bins + khi);
             }
             if (a.db && act) {
-                a.db[gc * a.bins + k] = fmaxf(a.db_alpha * log10f(ql), -120.f);
-                if (put_hi) a.db[gc * a.bins + khi] = fmaxf(a.db_alpha * log10f(qh), -120.f);
+                __builtin_nontemporal_store(fmaxf(a.db_alpha * log10f(ql), -120.f), a.db + gc * a.bins + k);
+                if (put_hi)
+                    __builtin_nontemporal_store(fmaxf(a.db_alpha * log10f(qh), -120.f), a.db + gc * a.bins + khi);
             }
         }
     }
@@ -107,7 +117,7 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
 // each) of channel c and sums |X[k]|^2, k <= N/2, over them in registers;
 // the TPW slots are then added through LDS in a fixed order and written as
 // partial[j * cols + c * (N/2 + 1) + k].
-template <int N, bool WIN>
+template <int N, bool WIN, bool PAIR>
 __global__ void __launch_bounds__(RCfg<N / 2>::threads)
 k_real_power(const float* __restrict__ sig, int64_t samples, int64_t hop, int64_t frames, int64_t wpc,
              int64_t iters, const float* __restrict__ window, float* __restrict__ partial, int64_t cols,
@@ -133,15 +143,7 @@ k_real_power(const float* __restrict__ sig, int64_t samples, int64_t hop, int64_
         const float* p = base + (act ? f : 0) * hop;
         float2 v[E];
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int q = 2 * (tl + T * m);
-            float x0 = p[q], x1 = p[q + 1];
-            if constexpr (WIN) {
-                x0 *= window[q];
-                x1 *= window[q + 1];
-            }
-            v[m] = make_float2(x0, x1);
-        }
+        for (int m = 0; m < E; ++m) v[m] = ld_pair<WIN, PAIR>(p, window, 2 * (tl + T * m));
         rt.template fft<true>(v, buf, tl, tab);
         lds_barrier();
 #pragma unroll
@@ -177,7 +179,7 @@ k_real_power(const float* __restrict__ sig, int64_t samples, int64_t hop, int64_
     }
 }
 
-template <int N>
+template <int N, bool PAIR>
 __global__ void __launch_bounds__(RCfg<N / 2>::threads)
 k_real_inverse(const float2* __restrict__ y, int64_t in_stride, float* __restrict__ x, int64_t batch,
                const float2* __restrict__ tab, const float2* __restrict__ roots) {
@@ -211,11 +213,18 @@ k_real_inverse(const float2* __restrict__ y, int64_t in_stride, float* __restric
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int j = 2 * (t + T * m);
-            out[j] = v[m].x * inv_m;
-            out[j + 1] = -v[m].y * inv_m;
+            const float2 y = make_float2(v[m].x * inv_m, -v[m].y * inv_m);
+            if constexpr (PAIR) {
+                gstore<true>(reinterpret_cast<float2*>(out + j), y);
+            } else {
+                __builtin_nontemporal_store(y.x, out + j);
+                __builtin_nontemporal_store(y.y, out + j + 1);
+            }
         }
     }
 }
+
+inline bool aligned8(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7) == 0; }
 
 template <int N>
 hipError_t frames_n(const RealFrames& a, hipStream_t st) {
@@ -223,10 +232,15 @@ hipError_t frames_n(const RealFrames& a, hipStream_t st) {
     const int64_t blocks = (a.total + C::TPW - 1) / C::TPW;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     const float2* roots = a.tab + real_roots_offset(N);
-    if (a.window)
-        hipLaunchKernelGGL((k_real_frames<N, true>), dim3((unsigned)blocks), dim3(C::threads), 0, st, a, roots);
-    else
-        hipLaunchKernelGGL((k_real_frames<N, false>), dim3((unsigned)blocks), dim3(C::threads), 0, st, a, roots);
+    const bool pair = aligned8(a.sig) && aligned8(a.window) && a.samples % 2 == 0 && a.hop % 2 == 0;
+    const dim3 grid((unsigned)blocks), block(C::threads);
+    if (a.window) {
+        if (pair) hipLaunchKernelGGL((k_real_frames<N, true, true>), grid, block, 0, st, a, roots);
+        else hipLaunchKernelGGL((k_real_frames<N, true, false>), grid, block, 0, st, a, roots);
+    } else {
+        if (pair) hipLaunchKernelGGL((k_real_frames<N, false, true>), grid, block, 0, st, a, roots);
+        else hipLaunchKernelGGL((k_real_frames<N, false, false>), grid, block, 0, st, a, roots);
+    }
     return hipGetLastError();
 }
 
@@ -237,14 +251,19 @@ hipError_t power_n(const RealPower& a, hipStream_t st) {
     if (blocks > 0x7fffffffLL || a.iters * a.wpc * TPW < a.frames) return hipErrorInvalidValue;
     const float2* roots = a.tab + real_roots_offset(N);
     const int64_t cols = a.channels * (N / 2 + 1);
-    if (a.window)
-        hipLaunchKernelGGL((k_real_power<N, true>), dim3((unsigned)blocks), dim3(RCfg<N / 2>::threads), 0, st,
-                           a.sig, a.samples, a.hop, a.frames, a.wpc, a.iters, a.window, a.partial, cols, a.tab,
-                           roots);
-    else
-        hipLaunchKernelGGL((k_real_power<N, false>), dim3((unsigned)blocks), dim3(RCfg<N / 2>::threads), 0,
-                           st, a.sig, a.samples, a.hop, a.frames, a.wpc, a.iters, a.window, a.partial, cols,
-                           a.tab, roots);
+    const bool pair = aligned8(a.sig) && aligned8(a.window) && a.samples % 2 == 0 && a.hop % 2 == 0;
+    const dim3 grid((unsigned)blocks), block(RCfg<N / 2>::threads);
+#define MIFFT_POW(W, P)                                                                                   \
+    hipLaunchKernelGGL((k_real_power<N, W, P>), grid, block, 0, st, a.sig, a.samples, a.hop, a.frames, a.wpc, \
+                       a.iters, a.window, a.partial, cols, a.tab, roots)
+    if (a.window) {
+        if (pair) MIFFT_POW(true, true);
+        else MIFFT_POW(true, false);
+    } else {
+        if (pair) MIFFT_POW(false, true);
+        else MIFFT_POW(false, false);
+    }
+#undef MIFFT_POW
     return hipGetLastError();
 }
 
@@ -254,8 +273,12 @@ hipError_t inverse_n(const float2* y, int64_t in_stride, float* x, int64_t batch
     using C = RCfg<N / 2>;
     const int64_t blocks = (batch + C::TPW - 1) / C::TPW;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_real_inverse<N>, dim3((unsigned)blocks), dim3(C::threads), 0, st, y, in_stride, x,
-                       batch, tab, tab + real_roots_offset(N));
+    if (aligned8(x))
+        hipLaunchKernelGGL((k_real_inverse<N, true>), dim3((unsigned)blocks), dim3(C::threads), 0, st, y, in_stride,
+                           x, batch, tab, tab + real_roots_offset(N));
+    else
+        hipLaunchKernelGGL((k_real_inverse<N, false>), dim3((unsigned)blocks), dim3(C::threads), 0, st, y,
+                           in_stride, x, batch, tab, tab + real_roots_offset(N));
     return hipGetLastError();
 }
 
