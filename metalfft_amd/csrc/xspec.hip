@@ -12,8 +12,8 @@
 //      real-input kernel (realfft.hip: an N/2-point complex FFT per frame and
 //      a split), bins 0..N/2 written to a spectra scratch [channel][frame]
 //      [N/2 + 1] (a real frame's other bins are their conjugates);
-//   2. partial sums over chunks of 64 frames, one thread per (series, bin,
-//      chunk), a series being a channel's power |X_c|^2 or a pair's cross
+//   2. partial sums over chunks of frames (16, or more for long signals),
+//      one workgroup per (chunk, 64 bins), of every series: a channel's power |X_c|^2, a pair's cross
 //      spectrum X_i conj(X_j) and per-frame coherence
 //      |S_ij|^2 / max(P_i P_j, 1e-30) (the kernel's definition, :56-61);
 //   3. the chunk partials added in a fixed order (deterministic), divided by
@@ -46,7 +46,14 @@ struct mifft_xspec {
 namespace mifft {
 namespace {
 
-constexpr int kChunk = 64;  // frames per partial sum
+// Frames per partial sum: at least 16, more when the partials of 16-frame
+// chunks would exceed 256 MiB or 4096 chunks.
+int64_t chunk_frames(int64_t frames, int64_t nser, int64_t nb) {
+    int64_t cap = ((int64_t)256 << 20) / (12 * nser * nb);
+    cap = cap < 1 ? 1 : (cap > 4096 ? 4096 : cap);
+    const int64_t len = (frames + cap - 1) / cap;
+    return len < 16 ? 16 : len;
+}
 
 // Pair p of the upper triangle -> (i, j).
 __device__ __forceinline__ void pair_of(int p, int c, int& i, int& j) {
@@ -64,40 +71,98 @@ __device__ __forceinline__ void pair_of(int p, int c, int& i, int& j) {
 // (chunk h, series s, bin k <= N/2): {sum re S, sum im S, sum coherence} for
 // pairs, {sum power, 0, 0} for channels, at partial[((h * nser + s) * n + k)
 // * 3], n = N/2 + 1 bins per spectrum.
+//
+// A workgroup is 64 bins x 4 groups over one chunk of frames: per frame the
+// spectra of all channels at its 64 bins are read once into LDS (each group
+// loads channels g, g + 4, ...; the next frame's loads are issued before the
+// current frame is used), then group g accumulates channel powers g, g + 4,
+// ... and pairs g, g + 4, ... from LDS.  Each spectrum value is read from
+// memory once, whatever the number of pairs.
+// KC: channel capacity (4, 8 or 16), sizing the accumulators.
+constexpr int kGroups = 4;
+
+template <int KC>
 __global__ void __launch_bounds__(256)
 k_xspec_partial(const float2* __restrict__ spec, float* __restrict__ partial, int64_t n, int c,
-                int64_t frames, int nser) {
-    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int s = blockIdx.y;
-    const int64_t h = blockIdx.z;
-    if (k >= n) return;
-    const int64_t f0 = h * kChunk;
-    const int64_t f1 = f0 + kChunk < frames ? f0 + kChunk : frames;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    if (s < c) {
-        const float2* x = spec + (int64_t)s * frames * n + k;
-        for (int64_t f = f0; f < f1; ++f) {
-            const float2 u = x[f * n];
-            a0 += u.x * u.x + u.y * u.y;
+                int64_t frames, int nser, int64_t chunk) {
+    constexpr int kMaxC = KC, kMaxP = KC * (KC - 1) / 2;
+    constexpr int kPairsPerGroup = (kMaxP + kGroups - 1) / kGroups;
+    constexpr int kChanPerGroup = KC / kGroups;
+    __shared__ float2 xs[kMaxC][64];
+    __shared__ int pi_[kMaxP], pj_[kMaxP];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t k = (int64_t)blockIdx.y * 64 + lane;
+    const int64_t kc = k < n ? k : n - 1;  // in-range bin for the loads
+    const int64_t h = blockIdx.x;
+    const int np = c * (c - 1) / 2;
+    for (int p = threadIdx.x; p < np; p += 256) pair_of(p, c, pi_[p], pj_[p]);
+    const int64_t f0 = h * chunk;
+    const int64_t f1 = f0 + chunk < frames ? f0 + chunk : frames;
+    float acc[kPairsPerGroup][3], pw[kChanPerGroup];
+#pragma unroll
+    for (int q = 0; q < kPairsPerGroup; ++q) acc[q][0] = acc[q][1] = acc[q][2] = 0.f;
+#pragma unroll
+    for (int q = 0; q < kChanPerGroup; ++q) pw[q] = 0.f;
+    // loads of channel min(ch, c - 1) at every frame: unconditional, rows of
+    // absent channels are never read back
+    float2 cur[kChanPerGroup], nxt[kChanPerGroup];
+#pragma unroll
+    for (int q = 0; q < kChanPerGroup; ++q) {
+        const int ch = g + kGroups * q < c ? g + kGroups * q : c - 1;
+        cur[q] = spec[((int64_t)ch * frames + f0) * n + kc];
+    }
+    for (int64_t f = f0; f < f1; ++f) {
+        const int64_t fn = f + 1 < f1 ? f + 1 : f;
+#pragma unroll
+        for (int q = 0; q < kChanPerGroup; ++q) {
+            const int ch = g + kGroups * q < c ? g + kGroups * q : c - 1;
+            nxt[q] = spec[((int64_t)ch * frames + fn) * n + kc];
         }
-    } else {
-        int i, j;
-        pair_of(s - c, c, i, j);
-        const float2* xi = spec + (int64_t)i * frames * n + k;
-        const float2* xj = spec + (int64_t)j * frames * n + k;
-        for (int64_t f = f0; f < f1; ++f) {
-            const float2 u = xi[f * n], w = xj[f * n];
-            const float2 sx = make_float2(u.x * w.x + u.y * w.y, u.y * w.x - u.x * w.y);  // u conj(w)
-            const float pi = u.x * u.x + u.y * u.y, pj = w.x * w.x + w.y * w.y;
-            a0 += sx.x;
-            a1 += sx.y;
-            a2 += (sx.x * sx.x + sx.y * sx.y) / fmaxf(pi * pj, 1e-30f);
+        __syncthreads();  // the previous frame's LDS reads (and the pair table) are done
+#pragma unroll
+        for (int q = 0; q < kChanPerGroup; ++q) xs[g + kGroups * q][lane] = cur[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kChanPerGroup; ++q) {
+            const float2 u = cur[q];
+            pw[q] += u.x * u.x + u.y * u.y;
+        }
+#pragma unroll
+        for (int q = 0; q < kPairsPerGroup; ++q) {
+            const int p = g + kGroups * q;
+            if (p < np) {
+                const float2 u = xs[pi_[p]][lane], w = xs[pj_[p]][lane];
+                const float2 sx = make_float2(u.x * w.x + u.y * w.y, u.y * w.x - u.x * w.y);  // u conj(w)
+                const float pi = u.x * u.x + u.y * u.y, pj = w.x * w.x + w.y * w.y;
+                acc[q][0] += sx.x;
+                acc[q][1] += sx.y;
+                acc[q][2] += (sx.x * sx.x + sx.y * sx.y) / fmaxf(pi * pj, 1e-30f);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kChanPerGroup; ++q) cur[q] = nxt[q];
+    }
+    if (k >= n) return;
+#pragma unroll
+    for (int q = 0; q < kChanPerGroup; ++q) {
+        const int ch = g + kGroups * q;
+        if (ch < c) {
+            float* o = partial + ((h * nser + ch) * n + k) * 3;
+            o[0] = pw[q];
+            o[1] = 0.f;
+            o[2] = 0.f;
         }
     }
-    float* o = partial + ((h * nser + s) * n + k) * 3;
-    o[0] = a0;
-    o[1] = a1;
-    o[2] = a2;
+#pragma unroll
+    for (int q = 0; q < kPairsPerGroup; ++q) {
+        const int p = g + kGroups * q;
+        if (p < np && c + p < nser) {
+            float* o = partial + ((h * nser + c + p) * n + k) * 3;
+            o[0] = acc[q][0];
+            o[1] = acc[q][1];
+            o[2] = acc[q][2];
+        }
+    }
 }
 
 // Outputs for all N bins: power[c][k], cross[p][k] (complex), coherence[p][k],
@@ -184,9 +249,9 @@ mifft_status xspec_run(mifft_xspec* x, const float* signals, int64_t samples, in
                        float* coherence, int64_t pairs, hipStream_t st) {
     const int64_t n = x->n, c = x->channels, nb = n / 2 + 1;
     const int64_t total = c * frames;
-    const int64_t chunks = (frames + kChunk - 1) / kChunk;
     const int nser = (int)(c + pairs);
-    if (chunks > 65535) return fail(MIFFT_ERR_INVALID_ARGUMENT, "too many frames");
+    const int64_t chunk = chunk_frames(frames, nser, nb);
+    const int64_t chunks = (frames + chunk - 1) / chunk;
     mifft_status s;
     if ((s = x->spectra.reserve(sizeof(float2) * (size_t)(total * nb))) != MIFFT_OK ||
         (s = x->partial.reserve(sizeof(float) * 3 * (size_t)(chunks * nser * nb))) != MIFFT_OK)
@@ -196,8 +261,14 @@ mifft_status xspec_run(mifft_xspec* x, const float* signals, int64_t samples, in
     const RealFrames a{signals, samples, hop, frames, total, window, spec, nullptr, nullptr, 0.f, nb,
                        x->plan->real_tw};
     if ((s = check_hip(real_frames_forward((int)n, a, st), "cross-spectral FFT")) != MIFFT_OK) return s;
-    hipLaunchKernelGGL(k_xspec_partial, dim3((unsigned)((nb + 255) / 256), (unsigned)nser, (unsigned)chunks),
-                       dim3(256), 0, st, spec, part, nb, (int)c, frames, nser);
+    const dim3 pgrid((unsigned)chunks, (unsigned)((nb + 63) / 64));
+    if (c <= 4)
+        hipLaunchKernelGGL(k_xspec_partial<4>, pgrid, dim3(256), 0, st, spec, part, nb, (int)c, frames, nser, chunk);
+    else if (c <= 8)
+        hipLaunchKernelGGL(k_xspec_partial<8>, pgrid, dim3(256), 0, st, spec, part, nb, (int)c, frames, nser, chunk);
+    else
+        hipLaunchKernelGGL(k_xspec_partial<16>, pgrid, dim3(256), 0, st, spec, part, nb, (int)c, frames, nser,
+                           chunk);
     hipLaunchKernelGGL(k_xspec_reduce, dim3((unsigned)((n + 255) / 256), (unsigned)nser), dim3(256), 0, st,
                        part, n, (int)c, chunks, nser, 1.0f / (float)frames, power,
                        reinterpret_cast<float2*>(cross), coherence);

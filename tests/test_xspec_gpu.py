@@ -14,7 +14,7 @@ def rnd(shape, seed):
 
 @pytest.mark.parametrize("n,hop,c,s", [(1024, 512, 3, 4096), (256, 100, 5, 9000), (64, 64, 1, 640),
                                        (4096, 1024, 4, 40000), (16384, 8192, 2, 50000),
-                                       (512, 7, 16, 3000)])
+                                       (512, 7, 16, 3000), (64, 1, 2, 100000), (128, 3, 7, 20000)])
 def test_cross_spectral_vs_fp64(gpu, oracle, n, hop, c, s):
     x = rnd((c, s), n + c)
     w = gpu.window_coefficients("hann", n)
