@@ -4,16 +4,19 @@
 // No reference counterpart: MetalFFT.forward(batch:) rejects uneven lengths
 // (Sources/MetalFFT/FFT/MetalFFT.swift:108-114).
 //
-// Work is cut into tiles of exactly 16384 complex elements: 1 transform of
-// 16384, 2 of 8192, ..., 256 of 64 (the last tile of a length may hold
-// fewer).  A tile is one workgroup-iteration of 1024 threads: length N uses
-// N/16 threads per transform (radix-16 passes, 16 values per thread) and
-// 16384/N transforms side by side in the same 136 KiB of LDS, so every tile
-// keeps all 1024 threads and all of LDS busy whatever its length.  Tiles are
-// sorted by length, largest first, and a grid of one workgroup per CU strides
-// over them.  The same kernel launched once per length over that length's
-// tiles is the one-launch-per-size baseline.  Both forms can be captured in
-// a hipGraph and replayed.
+// Work is cut into tiles: one transform of 16384 (1024 threads, 128 KiB of
+// LDS), one of 8192 (512 threads, 64 KiB), and for lengths up to 4096 tiles
+// of 4096 elements (256 threads, 32 KiB): 1 of 4096, 2 of 2048, ..., 64 of 64
+// (the last tile of a length may hold fewer).  Length N uses N/16 threads per
+// transform (radix-16 passes, 16 values per thread), so every tile keeps all
+// its threads and LDS busy whatever its length.  Tiles are sorted by length,
+// largest first; each tile class is one persistent launch whose grid is as
+// many workgroups as fit on the GPU at once (one per CU for 16384, two for
+// 8192, four for the rest), striding over the class's tiles.  Splitting the
+// classes lets small lengths run several workgroups per CU, so one
+// workgroup's loads overlap another's passes.  The same kernels launched
+// once per length over that length's tiles are the one-launch-per-size
+// baseline.  Both forms can be captured in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,9 +32,19 @@ namespace mifft {
 
 namespace {
 
-constexpr int kTileElems = 16384;
-constexpr int kTileThreads = kTileElems / 16;
-constexpr int kTileLds = kTileElems + kTileElems / 16;  // LdsPad::len(N) * (16384 / N) for all N
+int ragged_cus() {
+    static int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 256;
+        return n > 0 ? n : 256;
+    }();
+    return cus;
+}
+
+// Tile elements of length 2^l: 16384, 8192, else 4096.
+constexpr int tile_elems(int l) { return l >= 14 ? 16384 : (l == 13 ? 8192 : 4096); }
 
 // Twiddle tables of every length back to back: the plan table of N = 2^l
 // (W_N^k, k < N, then the packed bases rows of PassTable<N, 16, 0>) starts at
@@ -48,76 +61,103 @@ constexpr int table_len(int l) { return l == 6 ? 64 + PassTable<64, 16, 0>::size
 constexpr int table_off(int l) { return l <= 6 ? 0 : table_off(l - 1) + table_len(l - 1); }
 constexpr int kTablesLen = table_off(15);
 
-// The transform's data loads are issued first, then its packed twiddle
-// bases rows (one 32-byte row per butterfly and pass), so no pass waits on a
-// table read.
+// One transform of a tile: thread `tid` of the workgroup works on transform
+// tid / (N/16) at offset `off` (`active`: it exists).  The data loads are
+// unconditional (an inactive slot reads the tile's last transform and never
+// stores), non-temporal, and issued before the packed twiddle bases rows
+// (one 32-byte row per butterfly and pass), so no pass waits on a table read.
 template <int N, bool INV>
-__device__ __forceinline__ void tile_body(const float2* in, float2* out, const float2* tw,
-                                          const int64_t* offs, int count, float2* lds) {
+__device__ __forceinline__ void tile_body(const float2* in, float2* out, const float2* tw, int64_t off,
+                                          bool active, float2* lds) {
     constexpr int E = 16;
     constexpr int T = N / E;
     const int tid = fresh((int)threadIdx.x);  // index arithmetic redone per tile, not hoisted
     const int local = tid / T;
     const int t = tid % T;
-    const bool active = local < count;
-    const int64_t off = active ? offs[local] : 0;
-    float2* buf = lds + local * LdsPad::len(N);
+    float2* buf = lds + local * LdsSwz::len(N);
     float2 v[E];
-    load_column<N, E, INV, false>(v, in + off, t, active);
+    load_column<N, E, INV, true>(v, in + off, t, true);
     PreBases<N, E> pb;
     pb.load_packed(tw, t);
-    stockham_regs<N, E, LdsPad, 0, true, true, false, 2>(v, buf, t, tw, pb.w);
-    if (active) store_column<N, E, INV, false>(v, out + off, t);
+    stockham_regs<N, E, LdsSwz, 0, true, true, false, 2>(v, buf, t, tw, pb.w);
+    if (active) store_column<N, E, INV, true>(v, out + off, t);
 }
 
 // Workgroup b takes tiles tile_begin + b, + gridDim.x, ... of the sorted
 // list; the tiles of one length form a contiguous range, so that sequence is
 // walked as one loop per length (largest first), each with its own
-// transform code.
+// transform code.  The next tile's descriptor and this thread's offset in it
+// are loaded while the current tile runs (clamped indices, no branch).
 template <int L, bool INV>
 __device__ __forceinline__ void length_loop(const float2* in, float2* out, const float2* tw_all,
-                                            const RaggedTile* tiles, const int64_t* offs,
-                                            const RaggedRanges& rr, int tile_begin, int tile_end,
-                                            float2* lds) {
-    const int b = rr.lo[L - 6] > tile_begin ? rr.lo[L - 6] : tile_begin;
-    const int e = rr.hi[L - 6] < tile_end ? rr.hi[L - 6] : tile_end;
+                                            const RaggedTile* tiles, const int64_t* offs, int lo, int hi,
+                                            int tile_begin, int tile_end, float2* lds) {
+    constexpr int T = (1 << L) / 16;
+    const int b = lo > tile_begin ? lo : tile_begin;
+    const int e = hi < tile_end ? hi : tile_end;
     const int stride = (int)gridDim.x;
     const int r = tile_begin + (int)blockIdx.x;
-    for (int ti = b + (((r - b) % stride) + stride) % stride; ti < e; ti += stride) {
-        const RaggedTile d = tiles[ti];
-        tile_body<(1 << L), INV>(in, out, tw_all + table_off(L), offs + d.first, d.count, lds);
+    int ti = b + (((r - b) % stride) + stride) % stride;
+    if (ti >= e) return;
+    const int local = (int)threadIdx.x / T;
+    RaggedTile d = tiles[ti];
+    int64_t off = offs[d.first + (local < d.count ? local : d.count - 1)];
+    for (; ti < e; ti += stride) {
+        const int tn = ti + stride < e ? ti + stride : ti;
+        const RaggedTile dn = tiles[tn];
+        const int64_t offn = offs[dn.first + (local < dn.count ? local : dn.count - 1)];
+        tile_body<(1 << L), INV>(in, out, tw_all + table_off(L), off, local < d.count, lds);
+        d = dn;
+        off = offn;
     }
 }
 
-template <bool INV>
-__global__ void __launch_bounds__(kTileThreads)
+// One tile class: TILE elements per tile, TILE/16 threads, lengths
+// LMIN..LMAX.  LDS: TILE/N transforms of LdsSwz::len(N) = N.
+template <bool INV, int TILE, int LMIN, int LMAX>
+__global__ void __launch_bounds__(TILE / 16)
 k_ragged(const float2* in, float2* out, const float2* __restrict__ tw_all,
          const RaggedTile* __restrict__ tiles, const int64_t* __restrict__ offs, RaggedRanges rr,
          int tile_begin, int tile_end) {
-    __shared__ float2 lds[kTileLds];
-    length_loop<14, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<13, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<12, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<11, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<10, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<9, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<8, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<7, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
-    length_loop<6, INV>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, lds);
+    __shared__ float2 lds[TILE];
+    if constexpr (LMAX >= 14 && LMIN <= 14) length_loop<14, INV>(in, out, tw_all, tiles, offs, rr.lo[8], rr.hi[8], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 13 && LMIN <= 13) length_loop<13, INV>(in, out, tw_all, tiles, offs, rr.lo[7], rr.hi[7], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 12 && LMIN <= 12) length_loop<12, INV>(in, out, tw_all, tiles, offs, rr.lo[6], rr.hi[6], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 11 && LMIN <= 11) length_loop<11, INV>(in, out, tw_all, tiles, offs, rr.lo[5], rr.hi[5], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 10 && LMIN <= 10) length_loop<10, INV>(in, out, tw_all, tiles, offs, rr.lo[4], rr.hi[4], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 9 && LMIN <= 9) length_loop<9, INV>(in, out, tw_all, tiles, offs, rr.lo[3], rr.hi[3], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 8 && LMIN <= 8) length_loop<8, INV>(in, out, tw_all, tiles, offs, rr.lo[2], rr.hi[2], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 7 && LMIN <= 7) length_loop<7, INV>(in, out, tw_all, tiles, offs, rr.lo[1], rr.hi[1], tile_begin, tile_end, lds);
+    if constexpr (LMAX >= 6 && LMIN <= 6) length_loop<6, INV>(in, out, tw_all, tiles, offs, rr.lo[0], rr.hi[0], tile_begin, tile_end, lds);
+}
+
+// Launches the class kernel over tiles [tile_begin, tile_end) (all of one
+// class) with as many workgroups as are resident on the device at once.
+template <int TILE, int LMIN, int LMAX>
+hipError_t launch_class(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tiles,
+                        const int64_t* offs, const RaggedRanges& rr, int tile_begin, int tile_end,
+                        bool inverse, hipStream_t st) {
+    const int ntiles = tile_end - tile_begin;
+    if (ntiles <= 0) return hipSuccess;
+    static const int per_cu = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_ragged<false, TILE, LMIN, LMAX>, TILE / 16, 0) !=
+                hipSuccess ||
+            b < 1)
+            b = 1;
+        return b;
+    }();
+    const int grid = std::min(ntiles, ragged_cus() * per_cu);
+    if (inverse)
+        hipLaunchKernelGGL((k_ragged<true, TILE, LMIN, LMAX>), dim3(grid), dim3(TILE / 16), 0, st, in, out,
+                           tw_all, tiles, offs, rr, tile_begin, tile_end);
+    else
+        hipLaunchKernelGGL((k_ragged<false, TILE, LMIN, LMAX>), dim3(grid), dim3(TILE / 16), 0, st, in, out,
+                           tw_all, tiles, offs, rr, tile_begin, tile_end);
+    return hipGetLastError();
 }
 
 }  // namespace
-
-int ragged_cus() {
-    static int cus = [] {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess) return 256;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            return 256;
-        return n > 0 ? n : 256;
-    }();
-    return cus;
-}
 
 int ragged_twiddles_len() { return kTablesLen; }
 
@@ -129,20 +169,19 @@ hipError_t launch_ragged_twiddles(float2* tw_all, hipStream_t st) {
     return hipSuccess;
 }
 
+// Tiles [tile_begin, tile_end) of one class (16384, 8192 or up to 4096).
 hipError_t launch_ragged(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tiles,
-                         const int64_t* offs, const RaggedRanges& rr, int tile_begin, int tile_end,
+                         const int64_t* offs, const RaggedRanges& rr, int cls, int tile_begin, int tile_end,
                          bool inverse, hipStream_t st) {
-    const int ntiles = tile_end - tile_begin;
-    if (ntiles <= 0) return hipSuccess;
-    const int grid = std::min(ntiles, ragged_cus());
-    if (inverse)
-        hipLaunchKernelGGL(k_ragged<true>, dim3(grid), dim3(kTileThreads), 0, st, in, out, tw_all,
-                           tiles, offs, rr, tile_begin, tile_end);
-    else
-        hipLaunchKernelGGL(k_ragged<false>, dim3(grid), dim3(kTileThreads), 0, st, in, out, tw_all,
-                           tiles, offs, rr, tile_begin, tile_end);
-    return hipGetLastError();
+    switch (cls) {
+        case 0: return launch_class<16384, 14, 14>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, inverse, st);
+        case 1: return launch_class<8192, 13, 13>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, inverse, st);
+        default: return launch_class<4096, 6, 12>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, inverse, st);
+    }
 }
+
+// Class of length 2^l: 0 (16384), 1 (8192), 2 (up to 4096).
+int class_of(int l) { return l >= 14 ? 0 : (l == 13 ? 1 : 2); }
 
 }  // namespace mifft
 
@@ -155,12 +194,19 @@ mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool 
     const RaggedTile* tiles = static_cast<const RaggedTile*>(rp->d_tiles.ptr);
     const int64_t* offs = static_cast<const int64_t*>(rp->d_offsets.ptr);
     const float2* tw = static_cast<const float2*>(rp->d_twiddles.ptr);
-    if ((mode & MIFFT_RAGGED_PER_SIZE) == 0)
-        return check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, 0, rp->ntiles, inverse, st),
-                         "ragged fft");
+    if ((mode & MIFFT_RAGGED_PER_SIZE) == 0) {
+        for (int c = 0; c < 3; ++c) {
+            mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, c, rp->class_begin[c],
+                                                     rp->class_begin[c + 1], inverse, st),
+                                       "ragged fft");
+            if (r != MIFFT_OK) return r;
+        }
+        return MIFFT_OK;
+    }
     for (size_t s = 0; s + 1 < rp->size_tile_begin.size(); ++s) {
-        mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, rp->size_tile_begin[s],
-                                                 rp->size_tile_begin[s + 1], inverse, st),
+        const int l = rp->size_log2[s];
+        mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, class_of(l),
+                                                 rp->size_tile_begin[s], rp->size_tile_begin[s + 1], inverse, st),
                                    "ragged fft (per size)");
         if (r != MIFFT_OK) return r;
     }
@@ -208,14 +254,15 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
                      [&](int64_t a, int64_t b) { return lengths[a] > lengths[b]; });
     std::vector<int64_t> sorted_offs(count);
     std::vector<RaggedTile> tiles;
-    std::vector<int> begins;
+    std::vector<int> begins, log2s;
     int64_t i = 0;
     while (i < count) {
         const int64_t n = lengths[order[i]];
         int l = 0;
         while ((int64_t(1) << l) < n) ++l;
         begins.push_back((int)tiles.size());
-        const int per_tile = (int)(16384 / n);
+        log2s.push_back(l);
+        const int per_tile = (int)(tile_elems(l) / n);
         int64_t j = i;
         while (j < count && lengths[order[j]] == n) ++j;
         for (int64_t a = i; a < j; a += per_tile) {
@@ -235,10 +282,18 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
     rp->extent = extent;
     rp->ntiles = (int)tiles.size();
     rp->size_tile_begin = begins;
+    rp->size_log2 = log2s;
+    // class c covers the sorted tiles [class_begin[c], class_begin[c + 1])
+    for (int c = 0; c <= 3; ++c) rp->class_begin[c] = (int)tiles.size();
     for (size_t q = 0; q + 1 < begins.size(); ++q) {
-        const int l = tiles[(size_t)begins[q]].log2n;
+        const int l = log2s[q];
         rp->ranges.lo[l - 6] = begins[q];
         rp->ranges.hi[l - 6] = begins[q + 1];
+    }
+    for (int c = 2; c >= 0; --c) {  // classes in descending length order: 0, 1, 2
+        rp->class_begin[c] = rp->class_begin[c + 1];
+        for (size_t q = 0; q + 1 < begins.size(); ++q)
+            if (class_of(log2s[q]) == c && begins[q] < rp->class_begin[c]) rp->class_begin[c] = begins[q];
     }
     for (int64_t a = 0; a < count; ++a) rp->elements += lengths[a];
     if (hipStreamCreateWithFlags(&rp->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -285,7 +340,10 @@ int64_t mifft_ragged_elements(const mifft_ragged_plan* rp) { return rp ? rp->ele
 int64_t mifft_ragged_launches(const mifft_ragged_plan* rp, int mode) {
     if (!rp) return -1;
     if (rp->ntiles == 0) return 0;
-    return (mode & MIFFT_RAGGED_PER_SIZE) ? (int64_t)rp->size_tile_begin.size() - 1 : 1;
+    if (mode & MIFFT_RAGGED_PER_SIZE) return (int64_t)rp->size_tile_begin.size() - 1;
+    int64_t n = 0;
+    for (int c = 0; c < 3; ++c) n += rp->class_begin[c + 1] > rp->class_begin[c] ? 1 : 0;
+    return n;
 }
 
 mifft_status mifft_ragged_execute_device(mifft_ragged_plan* rp, const float* in, float* out,

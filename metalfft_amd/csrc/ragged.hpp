@@ -26,7 +26,9 @@ struct mifft_ragged_plan {
     int64_t extent = 0;    // max(offset + length): buffer length needed
     int ntiles = 0;
     std::vector<int> size_tile_begin;  // tile range per length, largest length first
+    std::vector<int> size_log2;        // log2 length of each of those ranges
     mifft::RaggedRanges ranges = {};   // the same ranges indexed by log2 length
+    int class_begin[4] = {0, 0, 0, 0};  // tile classes 16384, 8192, <= 4096: [class_begin[c], class_begin[c + 1])
     mifft::DeviceBuffer d_tiles, d_offsets, d_twiddles;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;

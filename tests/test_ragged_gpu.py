@@ -1,5 +1,6 @@
-"""Ragged batches (lengths 64 ... 16384 mixed) in one persistent launch, per
-size, and replayed from a hipGraph, against FP64 per transform."""
+"""Ragged batches (lengths 64 ... 16384 mixed) in one persistent launch per
+tile class, per size, and replayed from a hipGraph, against FP64 per
+transform."""
 import numpy as np
 import pytest
 
@@ -37,7 +38,8 @@ def test_ragged_small(gpu, oracle, mode):
     x = ragged_data(lengths, 1)
     rp = gpu.RaggedFFT(lengths)
     assert rp.extent == x.size and rp.elements == x.size
-    assert rp.launches(mode) == (1 if mode & 1 == 0 else len(set(lengths.tolist())))
+    classes = len({0 if n == 16384 else (1 if n == 8192 else 2) for n in lengths})
+    assert rp.launches(mode) == (classes if mode & 1 == 0 else len(set(lengths.tolist())))
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
     rp.execute(xd, yd, mode=mode)
@@ -47,6 +49,25 @@ def test_ragged_small(gpu, oracle, mode):
     rp.execute(yd, zd, direction=gpu.INVERSE, mode=mode)
     torch.cuda.synchronize()
     assert oracle.rel_l2(zd.cpu().numpy(), x) < 1e-6
+
+
+@pytest.mark.parametrize("lengths", [[64, 4096, 128, 1024] * 40, [8192, 64] * 30, [16384] * 5,
+                                     [16384, 8192] * 7, [2048] * 3 + [8192]])
+def test_ragged_tile_classes(gpu, oracle, lengths):
+    """Plans with one, two or three tile classes present, persistent and
+    per size."""
+    import torch
+    lengths = np.array(lengths, np.int64)
+    x = ragged_data(lengths, 5)
+    rp = gpu.RaggedFFT(lengths)
+    classes = len({0 if n == 16384 else (1 if n == 8192 else 2) for n in lengths})
+    assert rp.launches() == classes
+    xd = torch.from_numpy(x).cuda()
+    for mode in (0, 1):
+        yd = torch.empty_like(xd)
+        rp.execute(xd, yd, mode=mode)
+        torch.cuda.synchronize()
+        check_all(oracle, lengths, x, yd.cpu().numpy())
 
 
 def test_ragged_matches_uniform_plans(gpu, oracle):
