@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--kernel", default=None, help="pin a kernel variant by name")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the MFMA / ragged / convolve / SAR measurements")
+    p.add_argument("--only", default=None,
+                   help="comma-separated extras to run (default: all), e.g. stft_welch_2p24")
     p.add_argument("--traffic-probe", action="store_true",
                    help="only run 4 launches of a plain 128 MiB copy (elementwise, known "
                         "bytes) and 4 of the FFT step, for rocprofv3 --pmc FETCH_SIZE / "
@@ -543,6 +545,8 @@ def main():
                         ("cross_spectral_8ch", lambda: extra_xspec(m, torch)),
                         ("stft_welch_2p24", lambda: extra_stft(m, torch)),
                         ("real_fft_4096x4096", lambda: extra_real(m, torch))):
+            if a.only and key not in a.only.split(","):
+                continue
             try:
                 extras[key] = fn()
             except Exception as e:  # keep the headline line whatever happens here

@@ -97,32 +97,43 @@ constexpr int rows_threads() { return (N / rows_e<N>()) * tpw_for<N, rows_e<N>()
 //   out[r][f] = sum_{k=0..7} w[f][k] in[r + si[f] + k - 3][f],
 // taps outside the frame dropped (rda_kernels.metal:150-180).  The shift is
 // the same for every range bin, so si and the 8 windowed-sinc weights per
-// Doppler bin are formed once on the host in FP64.  A workgroup is 64
-// Doppler columns x 4 range bins: every tap load is one coalesced row
-// segment, and the workgroups of a column tile (same XCD, since ids
-// x + 64 y share x mod 8) reuse each other's rows from L2.
+// Doppler bin are formed once on the host in FP64.  A thread computes
+// kRcmcRows consecutive range bins of one column from a window of
+// kRcmcRows + 7 input rows held in registers (every input read once per
+// thread instead of once per tap); a workgroup is 64 Doppler columns x 4
+// such runs, so every load is one coalesced row segment.
+constexpr int kRcmcRows = 8;
+
 template <int IO>
 __global__ void __launch_bounds__(256)
 k_rcmc(const typename SarIo<IO>::T* __restrict__ in, typename SarIo<IO>::T* __restrict__ out,
        const int* __restrict__ si, const float4* __restrict__ w, int64_t nr, int64_t na) {
     const int64_t f = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-    const int64_t r = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (f >= na || r >= nr) return;
+    const int64_t r0 = ((int64_t)blockIdx.y * 4 + (threadIdx.x >> 6)) * kRcmcRows;
+    if (f >= na || r0 >= nr) return;
     const int s = si[f];
     const float4 wa = w[2 * f], wb = w[2 * f + 1];
     const float wk[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
-    float2 acc = make_float2(0.f, 0.f);
+    float2 x[kRcmcRows + 7];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int64_t rr = r + s + k - 3;
+    for (int q = 0; q < kRcmcRows + 7; ++q) {
+        const int64_t rr = r0 + s + q - 3;
         const bool ok = rr >= 0 && rr < nr;
-        const float2 x = SarIo<IO>::ld(in + (rr < 0 ? 0 : (rr >= nr ? nr - 1 : rr)) * na + f);
+        const float2 v = SarIo<IO>::ld(in + (rr < 0 ? 0 : (rr >= nr ? nr - 1 : rr)) * na + f);
         uint32_t m = ok ? 0xffffffffu : 0u;  // unconditional load, opaque mask
         __asm__("" : "+v"(m));
-        acc.x += wk[k] * __uint_as_float(__float_as_uint(x.x) & m);
-        acc.y += wk[k] * __uint_as_float(__float_as_uint(x.y) & m);
+        x[q] = make_float2(__uint_as_float(__float_as_uint(v.x) & m), __uint_as_float(__float_as_uint(v.y) & m));
     }
-    SarIo<IO>::st(out + r * na + f, acc);
+#pragma unroll
+    for (int j = 0; j < kRcmcRows; ++j) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            acc.x += wk[k] * x[j + k].x;
+            acc.y += wk[k] * x[j + k].y;
+        }
+        if (r0 + j < nr) SarIo<IO>::st(out + (r0 + j) * na + f, acc);
+    }
 }
 
 enum { kRowsCompress = 0, kRowsForward = 1, kRowsFilterInverse = 2 };
@@ -240,7 +251,7 @@ hipError_t rows_io(int io, int n, const void* in, void* out, const float2* tw, F
 
 hipError_t rcmc(int io, const void* in, void* out, const int* si, const float4* w, int64_t nr,
                 int64_t na, hipStream_t st) {
-    dim3 grid((unsigned)((na + 63) / 64), (unsigned)((nr + 3) / 4));
+    dim3 grid((unsigned)((na + 63) / 64), (unsigned)((nr + 4 * kRcmcRows - 1) / (4 * kRcmcRows)));
     if (io == MIFFT_IO_C32H)
         hipLaunchKernelGGL(k_rcmc<MIFFT_IO_C32H>, grid, dim3(256), 0, st, static_cast<const h2*>(in),
                            static_cast<h2*>(out), si, w, nr, na);

@@ -16,8 +16,8 @@
 //      one workgroup per (chunk, 64 bins), of every series: a channel's power |X_c|^2, a pair's cross
 //      spectrum X_i conj(X_j) and per-frame coherence
 //      |S_ij|^2 / max(P_i P_j, 1e-30) (the kernel's definition, :56-61);
-//   3. the chunk partials added in a fixed order (deterministic), divided by
-//      the frame count and written for all N bins (bin N - k from bin k,
+//   3. the chunk partials added by a 64-row tree in a fixed order
+//      (deterministic), divided by the frame count and written for all N bins (bin N - k from bin k,
 //      conjugated for the cross spectra); optionally the magnitude-squared
 //      coherence of the averaged spectra, |<S_ij>|^2 / (<P_i><P_j>), 0 where
 //      the denominator is not > 0 (PSD.coherence, Sources/MetalFFT/DSP/
@@ -88,7 +88,8 @@ k_xspec_partial(const float2* __restrict__ spec, float* __restrict__ partial, in
     constexpr int kMaxC = KC, kMaxP = KC * (KC - 1) / 2;
     constexpr int kPairsPerGroup = (kMaxP + kGroups - 1) / kGroups;
     constexpr int kChanPerGroup = KC / kGroups;
-    __shared__ float2 xs[kMaxC][64];
+    constexpr int D = KC == 16 ? 2 : 4;
+    __shared__ float2 xs[KC == 16 ? 2 : 4][kMaxC][64];
     __shared__ int pi_[kMaxP], pj_[kMaxP];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int64_t k = (int64_t)blockIdx.y * 64 + lane;
@@ -105,42 +106,54 @@ k_xspec_partial(const float2* __restrict__ spec, float* __restrict__ partial, in
     for (int q = 0; q < kChanPerGroup; ++q) pw[q] = 0.f;
     // loads of channel min(ch, c - 1) at every frame: unconditional, rows of
     // absent channels are never read back
-    float2 cur[kChanPerGroup], nxt[kChanPerGroup];
+    // D frames per step: their loads are issued one step ahead
+    float2 cur[D][kChanPerGroup], nxt[D][kChanPerGroup];
+    auto load = [&](float2 (&dst)[D][kChanPerGroup], int64_t fb) {
 #pragma unroll
-    for (int q = 0; q < kChanPerGroup; ++q) {
-        const int ch = g + kGroups * q < c ? g + kGroups * q : c - 1;
-        cur[q] = spec[((int64_t)ch * frames + f0) * n + kc];
-    }
-    for (int64_t f = f0; f < f1; ++f) {
-        const int64_t fn = f + 1 < f1 ? f + 1 : f;
+        for (int d = 0; d < D; ++d) {
+            const int64_t fr = fb + d < f1 ? fb + d : f1 - 1;
 #pragma unroll
-        for (int q = 0; q < kChanPerGroup; ++q) {
-            const int ch = g + kGroups * q < c ? g + kGroups * q : c - 1;
-            nxt[q] = spec[((int64_t)ch * frames + fn) * n + kc];
+            for (int q = 0; q < kChanPerGroup; ++q) {
+                const int ch = g + kGroups * q < c ? g + kGroups * q : c - 1;
+                dst[d][q] = spec[((int64_t)ch * frames + fr) * n + kc];
+            }
         }
-        __syncthreads();  // the previous frame's LDS reads (and the pair table) are done
+    };
+    load(cur, f0);
+    for (int64_t fb = f0; fb < f1; fb += D) {
+        load(nxt, fb + D);
+        __syncthreads();  // the previous step's LDS reads (and the pair table) are done
 #pragma unroll
-        for (int q = 0; q < kChanPerGroup; ++q) xs[g + kGroups * q][lane] = cur[q];
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int q = 0; q < kChanPerGroup; ++q) xs[d][g + kGroups * q][lane] = cur[d][q];
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < kChanPerGroup; ++q) {
-            const float2 u = cur[q];
-            pw[q] += u.x * u.x + u.y * u.y;
-        }
+        for (int d = 0; d < D; ++d) {
+            if (fb + d >= f1) break;  // uniform across the workgroup
 #pragma unroll
-        for (int q = 0; q < kPairsPerGroup; ++q) {
-            const int p = g + kGroups * q;
-            if (p < np) {
-                const float2 u = xs[pi_[p]][lane], w = xs[pj_[p]][lane];
-                const float2 sx = make_float2(u.x * w.x + u.y * w.y, u.y * w.x - u.x * w.y);  // u conj(w)
-                const float pi = u.x * u.x + u.y * u.y, pj = w.x * w.x + w.y * w.y;
-                acc[q][0] += sx.x;
-                acc[q][1] += sx.y;
-                acc[q][2] += (sx.x * sx.x + sx.y * sx.y) / fmaxf(pi * pj, 1e-30f);
+            for (int q = 0; q < kChanPerGroup; ++q) {
+                const float2 u = cur[d][q];
+                pw[q] += u.x * u.x + u.y * u.y;
+            }
+#pragma unroll
+            for (int q = 0; q < kPairsPerGroup; ++q) {
+                const int p = g + kGroups * q;
+                if (p < np) {
+                    const float2 u = xs[d][pi_[p]][lane], w = xs[d][pj_[p]][lane];
+                    const float2 sx = make_float2(u.x * w.x + u.y * w.y, u.y * w.x - u.x * w.y);  // u conj(w)
+                    const float pi = u.x * u.x + u.y * u.y, pj = w.x * w.x + w.y * w.y;
+                    acc[q][0] += sx.x;
+                    acc[q][1] += sx.y;
+                    // hardware reciprocal (1 ulp): the quotient is a per-frame coherence in [0, 1]
+                    acc[q][2] += (sx.x * sx.x + sx.y * sx.y) * __builtin_amdgcn_rcpf(fmaxf(pi * pj, 1e-30f));
+                }
             }
         }
 #pragma unroll
-        for (int q = 0; q < kChanPerGroup; ++q) cur[q] = nxt[q];
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int q = 0; q < kChanPerGroup; ++q) cur[d][q] = nxt[d][q];
     }
     if (k >= n) return;
 #pragma unroll
@@ -253,8 +266,9 @@ mifft_status xspec_run(mifft_xspec* x, const float* signals, int64_t samples, in
     const int64_t chunk = chunk_frames(frames, nser, nb);
     const int64_t chunks = (frames + chunk - 1) / chunk;
     mifft_status s;
+    const int64_t cols = 3 * nser * nb, lvl1 = (chunks + 63) / 64;
     if ((s = x->spectra.reserve(sizeof(float2) * (size_t)(total * nb))) != MIFFT_OK ||
-        (s = x->partial.reserve(sizeof(float) * 3 * (size_t)(chunks * nser * nb))) != MIFFT_OK)
+        (s = x->partial.reserve(sizeof(float) * (size_t)((chunks + 2 * lvl1 + 1) * cols))) != MIFFT_OK)
         return s;
     float2* spec = static_cast<float2*>(x->spectra.ptr);
     float* part = static_cast<float*>(x->partial.ptr);
@@ -269,8 +283,15 @@ mifft_status xspec_run(mifft_xspec* x, const float* signals, int64_t samples, in
     else
         hipLaunchKernelGGL(k_xspec_partial<16>, pgrid, dim3(256), 0, st, spec, part, nb, (int)c, frames, nser,
                            chunk);
+    // chunk partials summed by the 64-row tree (fixed order), then scaled and
+    // mirrored to all N bins
+    float* sums = part + (chunks + 2 * lvl1) * cols;
+    if ((s = check_hip(sum_rows(part, chunks, (int)cols, part + chunks * cols, part + (chunks + lvl1) * cols, sums,
+                                st),
+                       "cross-spectral sums")) != MIFFT_OK)
+        return s;
     hipLaunchKernelGGL(k_xspec_reduce, dim3((unsigned)((n + 255) / 256), (unsigned)nser), dim3(256), 0, st,
-                       part, n, (int)c, chunks, nser, 1.0f / (float)frames, power,
+                       sums, n, (int)c, (int64_t)1, nser, 1.0f / (float)frames, power,
                        reinterpret_cast<float2*>(cross), coherence);
     return check_hip(hipGetLastError(), "cross-spectral reduction");
 }
