@@ -331,13 +331,19 @@ def extra_convolve(m, torch, count=512, S=65536, K=1023, N=4096, cpu=True):
 
 def extra_sar(m, torch, n=4096):
     """Config 5 (range compression -> corner turn -> azimuth compression, no
-    RCMC) and the full RDA with RCMC, complex FP16, 4096 x 4096."""
+    RCMC) and the full RDA with RCMC, complex FP16 data, 4096 x 4096; the
+    full RDA also with range and azimuth compression in the three FP16
+    arithmetic modes."""
     g = torch.Generator(device="cuda").manual_seed(5)
     raw = (torch.randn((n, 2 * n), generator=g, device="cuda") * 0.1).half()
     img = torch.empty_like(raw)
     out = {"pulses": n, "range_samples": n, "io": "complex fp16"}
-    for key, rcmc, sweeps in (("config5_no_rcmc", 0, 3), ("full_rda_rcmc", 1, 5)):
-        rda = m.RangeDoppler(m.RdaParams.defaults(n_range=n, n_azimuth=n, io=1, rcmc=rcmc))
+    for key, rcmc, sweeps, prec in (("config5_no_rcmc", 0, 3, 0), ("full_rda_rcmc", 1, 5, 0),
+                                    ("full_rda_rcmc_fp16_pure", 1, 5, 1),
+                                    ("full_rda_rcmc_fp16_storage", 1, 5, 2),
+                                    ("full_rda_rcmc_fp16_mixed", 1, 5, 3)):
+        rda = m.RangeDoppler(m.RdaParams.defaults(n_range=n, n_azimuth=n, io=1, rcmc=rcmc,
+                                                  precision=prec))
         ms = time_fn(lambda: rda.process(raw, img), 10, 2, torch)
         nbytes = sweeps * 2 * 4 * n * n  # read + write sweeps of complex FP16
         out[key] = {"ms": ms, "frames_per_s": 1e3 / ms, "sweeps": sweeps,

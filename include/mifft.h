@@ -293,6 +293,11 @@ typedef struct mifft_rda_params {
     int filter_convention; /* MIFFT_RDA_MATCHED or MIFFT_RDA_REFERENCE_FORMULA */
     int rcmc;              /* 1: range cell migration correction between the azimuth FFT and
                               the azimuth filter (rda_kernels.metal:103-185), 0: none */
+    int precision;         /* mifft_conv_precision of range and azimuth compression: FP32, or
+                              the FP16 modes pure / storage / mixed of
+                              src/fft_sar_fused_fp16.metal:10-34 (4096 x 4096 frames only,
+                              else MIFFT_ERR_KERNEL_NOT_FOUND); the azimuth FFT before RCMC
+                              and RCMC stay FP32, as in precision_comparison.swift:226-230 */
 } mifft_rda_params;
 
 /* Azimuth filter phase sign and RCMC direction.  MATCHED: H_a = exp(+i phi),

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "convolve.hpp"
+#include "fp16_core.hpp"
 #include "runtime.hpp"
 #include "sar.hpp"
 #include "stockham_core.hpp"
@@ -42,7 +43,7 @@ namespace mifft {
 
 namespace {
 
-typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+using fp16::h2;
 
 template <int IO>
 struct SarIo;
@@ -189,6 +190,102 @@ k_rows(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2
     }
 }
 
+// FP16 precision modes of the rows (src/fft_sar_fused_fp16.metal: fused range
+// compression :250-806, fused azimuth compression :807-941, multiply + IFFT
+// :942-1059, each in modes pure / storage / mixed; N = 4096 as there), one
+// row per 256-thread workgroup, the mode arithmetic of the fused convolver:
+//   kRowsCompress       FFT (every pass / 16, so the spectrum is X / N and
+//                       FP16 does not overflow) -> x filter -> IFFT;
+//   kRowsFilterInverse  x filter -> IFFT (input: a stored spectrum).
+// The inverse is not normalised; the output is conj(.) times scale
+// (kRowsCompress) or scale / N (kRowsFilterInverse), powers of two, so
+// the result equals the FP32 rows' up to FP16 rounding.  The reference
+// divides by N only after the inverse and overflows FP16 for radar data
+// (its own "risk: overflow" note, fft_sar_fused_fp16.metal:18).
+template <int PM, int IO, class F, int MODE>
+__global__ void __launch_bounds__(256)
+k_rows_h(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2* __restrict__ tw,
+         F filt, float scale) {
+    using namespace fp16;
+    constexpr int N = 4096, T = 256;
+    __shared__ h2 lds[LdsPad::len(N)];
+    const int t = threadIdx.x;
+    const int64_t row = blockIdx.x;  // grid == rows
+    ConvReg<PM> v[16];
+    const typename SarIo<IO>::T* src = in + row * N;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const float2 x = SarIo<IO>::ld(src + t + T * m);
+        if constexpr (PM == kConvPure) v[m] = to_h2(x);
+        else v[m] = x;
+    }
+    PreBases<N, 16> pb;
+    pb.load_packed(tw, t);
+    if constexpr (MODE == kRowsCompress) conv_passes_h<PM, 0, false, true>(v, lds, t, pb.w);
+    {
+        const int tf = fresh(t);  // filter index arithmetic not shared with the loads
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const float2 hm = filt.at(row, tf + T * m);
+            if constexpr (PM == kConvPure) v[m] = hconj(hmul(v[m], to_h2(hm)));
+            else if constexpr (PM == kConvMixed) v[m] = to_f2(hconj(hmul(to_h2(v[m]), to_h2(hm))));
+            else v[m] = conjf2(round_h(cmul(round_h(v[m]), hm)));
+        }
+    }
+    pb.opaque();
+    conv_passes_h<PM, 0, MODE == kRowsCompress, false>(v, lds, t, pb.w);
+    const float s = MODE == kRowsCompress ? scale : scale / (float)N;
+    typename SarIo<IO>::T* dst = out + row * N;
+    const int ts = fresh(t);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        float2 r;  // conj(.) * s
+        if constexpr (PM == kConvPure) {
+            const _Float16 sh = (_Float16)s;
+            r = to_f2((h2){v[m].x * sh, -v[m].y * sh});
+        } else {
+            r = make_float2(v[m].x * s, -v[m].y * s);
+        }
+        SarIo<IO>::st(dst + ts + T * m, r);
+    }
+}
+
+template <int IO, class F, int MODE>
+hipError_t rows_h_io(int precision, const void* in, void* out, const float2* tw, F f, int64_t rows, float scale,
+                     hipStream_t st) {
+    using namespace fp16;
+    if (rows <= 0) return hipSuccess;
+    if (rows > 0x7fffffffLL) return hipErrorInvalidValue;
+    using Tp = typename SarIo<IO>::T;
+    const Tp* i = static_cast<const Tp*>(in);
+    Tp* o = static_cast<Tp*>(out);
+    switch (precision) {
+        case kConvPure:
+            hipLaunchKernelGGL((k_rows_h<kConvPure, IO, F, MODE>), dim3((unsigned)rows), dim3(256), 0, st, i, o, tw,
+                               f, scale);
+            break;
+        case kConvStorage:
+            hipLaunchKernelGGL((k_rows_h<kConvStorage, IO, F, MODE>), dim3((unsigned)rows), dim3(256), 0, st, i, o,
+                               tw, f, scale);
+            break;
+        case kConvMixed:
+            hipLaunchKernelGGL((k_rows_h<kConvMixed, IO, F, MODE>), dim3((unsigned)rows), dim3(256), 0, st, i, o, tw,
+                               f, scale);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Rows of length 4096 in an FP16 mode (precision 1-3).
+template <class F, int MODE = kRowsCompress>
+hipError_t rows_h(int precision, int io, const void* in, void* out, const float2* tw, F f, int64_t rows,
+                  float scale, hipStream_t st) {
+    return io == MIFFT_IO_C32H ? rows_h_io<MIFFT_IO_C32H, F, MODE>(precision, in, out, tw, f, rows, scale, st)
+                               : rows_h_io<MIFFT_IO_C64, F, MODE>(precision, in, out, tw, f, rows, scale, st);
+}
+
 // out[c][r] = in[r][c] for a rows x cols matrix, 64 x 64 tiles via LDS.
 template <int IO>
 __global__ void __launch_bounds__(256)
@@ -282,6 +379,10 @@ int ceil_log2(double x) {
 
 mifft_status rda_range(const mifft_rda* r, const void* raw, void* out, hipStream_t st) {
     RangeFilter f{static_cast<const float2*>(r->range_filter.ptr)};
+    if (r->p.precision != MIFFT_CONV_FP32)
+        return check_hip(rows_h(r->p.precision, r->p.io, raw, out, r->range_plan->twiddles, f, r->p.n_azimuth,
+                                (float)r->range_scale, st),
+                         "range compression (FP16)");
     return check_hip(rows_io(r->p.io, (int)r->p.n_range, raw, out, r->range_plan->twiddles, f,
                              r->p.n_azimuth, (float)r->range_scale, st),
                      "range compression");
@@ -295,10 +396,16 @@ mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStrea
     AzimuthFilter f{static_cast<const double*>(r->az_a.ptr), static_cast<const double*>(r->az_d.ptr),
                     r->az_b, r->p.n_range / 2,
                     r->p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0f : 1.0f};
-    if (!r->p.rcmc)
+    const bool half = r->p.precision != MIFFT_CONV_FP32;
+    if (!r->p.rcmc) {
+        if (half)
+            return check_hip(rows_h(r->p.precision, r->p.io, in, out, r->az_plan->twiddles, f, r->p.n_range,
+                                    (float)r->az_scale, st),
+                             "azimuth compression (FP16)");
         return check_hip(rows_io(r->p.io, (int)r->p.n_azimuth, in, out, r->az_plan->twiddles, f,
                                  r->p.n_range, (float)r->az_scale, st),
                          "azimuth compression");
+    }
     // azimuth FFT (scaled: the range-Doppler frame is stored, FP16 must not
     // overflow) into the work frame, RCMC from it into out, then filter +
     // IFFT in place
@@ -312,6 +419,10 @@ mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStrea
                             st),
                        "RCMC")) != MIFFT_OK)
         return s;
+    if (half)
+        return check_hip(rows_h<AzimuthFilter, kRowsFilterInverse>(r->p.precision, r->p.io, out, out,
+                                                                   r->az_plan->twiddles, f, r->p.n_range, 1.0f, st),
+                         "azimuth filter + IFFT (FP16)");
     return check_hip(rows_io<AzimuthFilter, kRowsFilterInverse>(r->p.io, (int)r->p.n_azimuth, out, out,
                                                                  r->az_plan->twiddles, f, r->p.n_range,
                                                                  1.0f, st),
@@ -459,6 +570,7 @@ mifft_status mifft_rda_default_params(mifft_rda_params* p) {
     p->io = MIFFT_IO_C32H;
     p->filter_convention = MIFFT_RDA_MATCHED;
     p->rcmc = 1;
+    p->precision = MIFFT_CONV_FP32;
     return MIFFT_OK;
 }
 
@@ -477,6 +589,11 @@ mifft_status mifft_rda_create(const mifft_rda_params* params, mifft_rda** out) {
     if (p.filter_convention != MIFFT_RDA_MATCHED && p.filter_convention != MIFFT_RDA_REFERENCE_FORMULA)
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown filter convention");
     if (p.rcmc != 0 && p.rcmc != 1) return fail(MIFFT_ERR_INVALID_ARGUMENT, "rcmc must be 0 or 1");
+    if (p.precision < MIFFT_CONV_FP32 || p.precision > MIFFT_CONV_FP16_MIXED)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown precision");
+    if (p.precision != MIFFT_CONV_FP32 && (p.n_range != 4096 || p.n_azimuth != 4096))
+        return fail(MIFFT_ERR_KERNEL_NOT_FOUND,
+                    "HIP kernel not found: the FP16 SAR modes are built for 4096 x 4096 frames");
     if (!(p.bandwidth > 0 && p.pulse_duration > 0 && p.sampling_rate > 0 && p.prf > 0 &&
           p.velocity > 0 && p.carrier_freq > 0 && p.center_range > 0))
         return fail(MIFFT_ERR_PRECONDITION, "SAR parameters must be positive");

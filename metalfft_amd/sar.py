@@ -24,7 +24,8 @@ class RdaParams(ctypes.Structure):
                 ("sampling_rate", ctypes.c_double), ("prf", ctypes.c_double),
                 ("velocity", ctypes.c_double), ("carrier_freq", ctypes.c_double),
                 ("center_range", ctypes.c_double), ("io", ctypes.c_int),
-                ("filter_convention", ctypes.c_int), ("rcmc", ctypes.c_int)]
+                ("filter_convention", ctypes.c_int), ("rcmc", ctypes.c_int),
+                ("precision", ctypes.c_int)]
 
     @classmethod
     def defaults(cls, **overrides) -> "RdaParams":

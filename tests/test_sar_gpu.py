@@ -115,3 +115,70 @@ def test_sar_fp16_config_4096(gpu, oracle):
     print(f"4096x4096 fp16 rel_l2={err:.3e} scales={rda.range_scale},{rda.azimuth_scale}")
     assert err < 3e-3
     assert np.isfinite(got).all()
+
+
+@pytest.fixture(scope="module")
+def scene_4096(oracle):
+    """The 4096 x 4096, 32-target scene as complex FP16 raw data, its FP64
+    RDA image (with RCMC) and the targets."""
+    import torch
+    from metalfft_amd.sar import RdaParams, RangeDoppler
+    p = oracle.sar_default_params()
+    tg = oracle.sar_targets(p, 32, 11)
+    raw = oracle.sar_simulate(p, tg)
+    rng = np.random.default_rng(7)
+    raw += 0.1 * (rng.standard_normal(raw.shape) + 1j * rng.standard_normal(raw.shape))
+    src = torch.from_numpy(raw.astype(np.complex64).view(np.float32)).cuda().to(torch.float16)
+    raw_used = src.float().cpu().numpy().view(np.complex64).astype(np.complex128)
+    rda = RangeDoppler(RdaParams.defaults(io=1))
+    ref = oracle.f64_rda(raw_used, p, rda.range_scale, rda.azimuth_scale, rcmc=True)
+    return p, tg, src, ref
+
+
+@pytest.mark.parametrize("precision,bound", [(1, 6e-3), (2, 4e-3), (3, 5e-3)])
+def test_sar_fp16_modes(gpu, oracle, scene_4096, precision, bound):
+    """Range and azimuth compression in the reference's FP16 modes (pure,
+    storage, mixed; fft_sar_fused_fp16.metal) on the 4096 x 4096 scene: the
+    image stays within the mode's bound of the FP64 pipeline, more than FP16
+    rounding away from it (the mode really computes in FP16), and every
+    target focuses on the same pixel as in FP32."""
+    import torch
+    from metalfft_amd.sar import RdaParams, RangeDoppler
+    p, tg, src, ref = scene_4096
+    imgs = {}
+    for prec in (0, precision):
+        rda = RangeDoppler(RdaParams.defaults(io=1, precision=prec))
+        img = torch.empty_like(src)
+        rda.process(src, img)
+        torch.cuda.synchronize()
+        imgs[prec] = img.float().cpu().numpy().view(np.complex64)
+    err = oracle.rel_l2(imgs[precision], ref)
+    err32 = oracle.rel_l2(imgs[0], ref)
+    print(f"precision={precision} rel_l2={err:.3e} (FP32 arithmetic: {err32:.3e})")
+    assert np.isfinite(imgs[precision]).all()
+    assert err < bound and err > err32
+    m16, m32 = np.abs(imgs[precision]), np.abs(imgs[0])
+    for r0, xt, _a in tg:
+        rb, ab = oracle.sar_expected_bins(p, r0, xt)
+        ri, ai = int(round(rb)), int(round(ab))
+        w16, w32 = m16[ri - 3:ri + 4, ai - 8:ai + 9], m32[ri - 3:ri + 4, ai - 8:ai + 9]
+        assert np.unravel_index(np.argmax(w16), w16.shape)[0] == np.unravel_index(np.argmax(w32), w32.shape)[0]
+
+
+def test_sar_fp16_mode_without_rcmc_and_errors(gpu, oracle, scene_4096):
+    import torch
+    from metalfft_amd.sar import RdaParams, RangeDoppler
+    p, tg, src, _ = scene_4096
+    out = {}
+    for prec in (0, 2):
+        rda = RangeDoppler(RdaParams.defaults(io=1, precision=prec, rcmc=0))
+        img = torch.empty_like(src)
+        rda.process(src, img)
+        torch.cuda.synchronize()
+        out[prec] = img.float().cpu().numpy().view(np.complex64)
+    assert oracle.rel_l2(out[2], out[0]) < 1e-2
+    with pytest.raises(gpu.FFTError) as e:
+        RangeDoppler(RdaParams.defaults(n_range=2048, precision=1))
+    assert e.value.case == "kernelNotFound"
+    with pytest.raises(gpu.FFTError):
+        RangeDoppler(RdaParams.defaults(precision=7))
