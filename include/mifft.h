@@ -398,6 +398,20 @@ MIFFT_API mifft_status mifft_welch_device(mifft_xspec *x, const float *signals, 
                                           int64_t hop, const float *window, double sample_rate,
                                           float *psd, void *stream);
 
+/* Blocking host-buffer forms of the three analyses above (the reference's
+ * calls are synchronous): inputs and outputs are host arrays of the same
+ * layouts; the analyser stages them through device memory on its own
+ * stream.  A window of NULL is rectangular. */
+MIFFT_API mifft_status mifft_xspec_host(mifft_xspec *x, const float *signals, int64_t samples,
+                                        int64_t hop, const float *window, float *power,
+                                        float *cross, float *coherence, float *welch_msc);
+MIFFT_API mifft_status mifft_stft_host(mifft_xspec *x, const float *signals, int64_t samples,
+                                       int64_t hop, const float *window, float *spectra,
+                                       float *power, float *power_db, float db_alpha);
+MIFFT_API mifft_status mifft_welch_host(mifft_xspec *x, const float *signals, int64_t samples,
+                                        int64_t hop, const float *window, double sample_rate,
+                                        float *psd);
+
 /* ------------------------------------------------------------ diagnostic */
 /* No counterpart in the reference.  Copies `bytes` from src to dst (device
  * memory, not overlapping) with the access style of the FFT kernels

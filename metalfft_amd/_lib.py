@@ -122,6 +122,10 @@ _SIGNATURES = [
                                   c_void_p, c_void_p, ctypes.c_float, c_void_p]),
     ("mifft_welch_device", c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p,
                                    ctypes.c_double, c_void_p, c_void_p]),
+    ("mifft_xspec_host", c_int, [c_void_p, fptr, c_int64, c_int64, fptr, fptr, fptr, fptr, fptr]),
+    ("mifft_stft_host", c_int, [c_void_p, fptr, c_int64, c_int64, fptr, fptr, fptr, fptr,
+                                ctypes.c_float]),
+    ("mifft_welch_host", c_int, [c_void_p, fptr, c_int64, c_int64, fptr, ctypes.c_double, fptr]),
 ]
 
 _lib = None

@@ -41,6 +41,7 @@ struct mifft_xspec {
     int64_t channels = 0;
     mifft_plan* plan = nullptr;                 // real-input table of length n
     mifft::DeviceBuffer spectra, partial, aux;  // scratch, grown on demand
+    mifft::DeviceBuffer hin, hout;              // staging of the host entry points
 };
 
 namespace mifft {
@@ -326,6 +327,8 @@ mifft_status mifft_xspec_destroy(mifft_xspec* x) {
     x->spectra.release();
     x->partial.release();
     x->aux.release();
+    x->hin.release();
+    x->hout.release();
     mifft_plan_destroy(x->plan);
     delete x;
     return MIFFT_OK;
@@ -413,6 +416,123 @@ mifft_status mifft_welch_device(mifft_xspec* x, const float* signals, int64_t sa
     hipLaunchKernelGGL(k_psd_finish, dim3((unsigned)((c * n + 255) / 256)), dim3(256), 0, st, sums, sumsq, n,
                        c * n, (float)(1.0 / ((double)frames * sample_rate)), psd);
     return check_hip(hipGetLastError(), "Welch PSD");
+}
+
+// ---- host entry points: stage through device buffers on the analyser's
+// stream, run the device form, copy back, block (the reference's calls are
+// synchronous).
+
+mifft_status mifft_xspec_host(mifft_xspec* x, const float* signals, int64_t samples, int64_t hop,
+                              const float* window, float* power, float* cross, float* coherence,
+                              float* welch_msc) {
+    if (!x) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null analyser");
+    if (samples < 0) return fail(MIFFT_ERR_INVALID_ARGUMENT, "negative size");
+    if (hop < 1) return fail(MIFFT_ERR_PRECONDITION, "hopSize must be >= 1");
+    const int64_t n = x->n, c = x->channels, pairs = c * (c - 1) / 2;
+    if (!power || (pairs > 0 && (!cross || !coherence)) || (samples > 0 && !signals))
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "null buffer");
+    hipStream_t st = x->plan->stream;
+    const size_t in_f = (size_t)(c * samples), w_f = window ? (size_t)n : 0;
+    const size_t o_pw = (size_t)(c * n), o_cr = (size_t)(2 * pairs * n), o_co = (size_t)(pairs * n);
+    const size_t o_ms = welch_msc ? (size_t)(pairs * n) : 0;
+    mifft_status s;
+    if ((s = x->hin.reserve(4 * (in_f + w_f + 1))) != MIFFT_OK ||
+        (s = x->hout.reserve(4 * (o_pw + o_cr + o_co + o_ms + 1))) != MIFFT_OK)
+        return s;
+    float* di = static_cast<float*>(x->hin.ptr);
+    float* dout = static_cast<float*>(x->hout.ptr);
+    if (in_f && (s = check_hip(hipMemcpyAsync(di, signals, 4 * in_f, hipMemcpyHostToDevice, st), "hipMemcpyAsync")) !=
+                    MIFFT_OK)
+        return s;
+    if (w_f && (s = check_hip(hipMemcpyAsync(di + in_f, window, 4 * w_f, hipMemcpyHostToDevice, st),
+                              "hipMemcpyAsync")) != MIFFT_OK)
+        return s;
+    float* dpw = dout;
+    float* dcr = dpw + o_pw;
+    float* dco = dcr + o_cr;
+    float* dms = dco + o_co;
+    if ((s = mifft_xspec_device(x, in_f ? di : nullptr, samples, hop, w_f ? di + in_f : nullptr, dpw,
+                                pairs ? dcr : nullptr, pairs ? dco : nullptr, o_ms ? dms : nullptr, st)) != MIFFT_OK)
+        return s;
+    struct Out {
+        float* h;
+        const float* d;
+        size_t n;
+    } outs[] = {{power, dpw, o_pw}, {cross, dcr, o_cr}, {coherence, dco, o_co}, {welch_msc, dms, o_ms}};
+    for (const Out& o : outs)
+        if (o.h && o.n && (s = check_hip(hipMemcpyAsync(o.h, o.d, 4 * o.n, hipMemcpyDeviceToHost, st),
+                                         "hipMemcpyAsync")) != MIFFT_OK)
+            return s;
+    return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+mifft_status mifft_stft_host(mifft_xspec* x, const float* signals, int64_t samples, int64_t hop,
+                             const float* window, float* spectra, float* power, float* power_db,
+                             float db_alpha) {
+    if (!x) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null analyser");
+    if (samples < 0) return fail(MIFFT_ERR_INVALID_ARGUMENT, "negative size");
+    if (hop < 1) return fail(MIFFT_ERR_PRECONDITION, "hopSize must be >= 1");
+    const int64_t frames = mifft_xspec_frames(x, samples, hop);
+    if (frames == 0) return MIFFT_OK;
+    if (!signals || !spectra) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null buffer");
+    hipStream_t st = x->plan->stream;
+    const int64_t n = x->n, c = x->channels;
+    const size_t in_f = (size_t)(c * samples), w_f = window ? (size_t)n : 0;
+    const size_t bins = (size_t)(c * frames * n);
+    const size_t o_pw = power ? bins : 0, o_db = power_db ? bins : 0;
+    mifft_status s;
+    if ((s = x->hin.reserve(4 * (in_f + w_f + 1))) != MIFFT_OK ||
+        (s = x->hout.reserve(4 * (2 * bins + o_pw + o_db))) != MIFFT_OK)
+        return s;
+    float* di = static_cast<float*>(x->hin.ptr);
+    float* dsp = static_cast<float*>(x->hout.ptr);
+    float* dpw = dsp + 2 * bins;
+    float* ddb = dpw + o_pw;
+    if ((s = check_hip(hipMemcpyAsync(di, signals, 4 * in_f, hipMemcpyHostToDevice, st), "hipMemcpyAsync")) !=
+            MIFFT_OK ||
+        (w_f && (s = check_hip(hipMemcpyAsync(di + in_f, window, 4 * w_f, hipMemcpyHostToDevice, st),
+                               "hipMemcpyAsync")) != MIFFT_OK))
+        return s;
+    if ((s = mifft_stft_device(x, di, samples, hop, w_f ? di + in_f : nullptr, dsp, o_pw ? dpw : nullptr,
+                               o_db ? ddb : nullptr, db_alpha, st)) != MIFFT_OK)
+        return s;
+    if ((s = check_hip(hipMemcpyAsync(spectra, dsp, 8 * bins, hipMemcpyDeviceToHost, st), "hipMemcpyAsync")) !=
+            MIFFT_OK ||
+        (o_pw && (s = check_hip(hipMemcpyAsync(power, dpw, 4 * bins, hipMemcpyDeviceToHost, st),
+                                "hipMemcpyAsync")) != MIFFT_OK) ||
+        (o_db && (s = check_hip(hipMemcpyAsync(power_db, ddb, 4 * bins, hipMemcpyDeviceToHost, st),
+                                "hipMemcpyAsync")) != MIFFT_OK))
+        return s;
+    return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+mifft_status mifft_welch_host(mifft_xspec* x, const float* signals, int64_t samples, int64_t hop,
+                              const float* window, double sample_rate, float* psd) {
+    if (!x) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null analyser");
+    if (samples < 0) return fail(MIFFT_ERR_INVALID_ARGUMENT, "negative size");
+    if (!psd || (samples > 0 && !signals)) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null buffer");
+    hipStream_t st = x->plan->stream;
+    const int64_t n = x->n, c = x->channels;
+    const size_t in_f = (size_t)(c * samples), w_f = window ? (size_t)n : 0;
+    mifft_status s;
+    if ((s = x->hin.reserve(4 * (in_f + w_f + 1))) != MIFFT_OK ||
+        (s = x->hout.reserve(4 * (size_t)(c * n))) != MIFFT_OK)
+        return s;
+    float* di = static_cast<float*>(x->hin.ptr);
+    if (in_f && (s = check_hip(hipMemcpyAsync(di, signals, 4 * in_f, hipMemcpyHostToDevice, st), "hipMemcpyAsync")) !=
+                    MIFFT_OK)
+        return s;
+    if (w_f && (s = check_hip(hipMemcpyAsync(di + in_f, window, 4 * w_f, hipMemcpyHostToDevice, st),
+                              "hipMemcpyAsync")) != MIFFT_OK)
+        return s;
+    float* dpsd = static_cast<float*>(x->hout.ptr);
+    if ((s = mifft_welch_device(x, in_f ? di : nullptr, samples, hop, w_f ? di + in_f : nullptr, sample_rate,
+                                dpsd, st)) != MIFFT_OK)
+        return s;
+    if ((s = check_hip(hipMemcpyAsync(psd, dpsd, 4 * (size_t)(c * n), hipMemcpyDeviceToHost, st),
+                       "hipMemcpyAsync")) != MIFFT_OK)
+        return s;
+    return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
 }  // extern "C"

@@ -5,12 +5,18 @@
 //   mifft::Convolver        Convolver           Sources/MetalFFT/DSP/Convolve.swift:8-87
 //   mifft::FusedConvolver   FusedConvolver      Sources/MetalFFT/DSP/FusedConvolver.swift:17-134
 //   mifft::Correlator       Correlator          Sources/MetalFFT/DSP/Correlate.swift:7-62
+//   mifft::WindowType       WindowType          Sources/MetalFFT/DSP/Window.swift:6-24
+//   mifft::STFT, STFTFrame  STFT                Sources/MetalFFT/DSP/STFT.swift:5-99
+//   mifft::FFTAnalyzer      FFTAnalyzer         Sources/MetalFFT/DSP/FFTAnalyzer.swift:66-116
+//   mifft::PSD              PSD                 Sources/MetalFFT/DSP/PSD.swift:6-134,
+//                                               CrossSpectral.swift:11-173
 //   mifft::FFTError         FFTError            Sources/MetalFFT/Internal/MetalContext.swift:6-30
 //
 // Methods throw FFTError where the Swift ones throw; complex values are
 // std::complex<float> (same interleaved layout as SIMD2<Float>).
 #pragma once
 
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <optional>
@@ -183,6 +189,218 @@ class Correlator {
 
    private:
     mifft_correlator* c_ = nullptr;
+};
+
+// WindowType.coefficients (Window.swift:14-23): the vDSP windows from their
+// published definitions (periodic, denominator N), the reference's flat-top
+// sum and Kaiser.
+struct WindowType {
+    enum Kind { hann, hamming, blackman, flatTop, kaiser, rectangular } kind = hann;
+    double beta = 6.0;  // kaiser only
+    static WindowType Kaiser(double beta) { return WindowType{kaiser, beta}; }
+    std::vector<float> coefficients(int64_t n) const {
+        std::vector<float> w((size_t)n);
+        const double pi = 3.14159265358979323846;
+        auto i0 = [](double x) {  // power series of I0, as Window.swift:99-110
+            double r = 1.0, t = 1.0, h = x / 2.0;
+            for (int k = 1; k <= 30; ++k) {
+                t *= h * h / (double)(k * k);
+                r += t;
+                if (t < 1e-12 * r) break;
+            }
+            return r;
+        };
+        for (int64_t k = 0; k < n; ++k) {
+            const double x = (double)k / (double)n;
+            double v = 1.0;
+            switch (kind) {
+                case hann: v = 0.5 * (1.0 - std::cos(2 * pi * x)); break;
+                case hamming: v = 0.54 - 0.46 * std::cos(2 * pi * x); break;
+                case blackman: v = 0.42 - 0.5 * std::cos(2 * pi * x) + 0.08 * std::cos(4 * pi * x); break;
+                case flatTop:
+                    v = 0.21557895 - 0.41663158 * std::cos(2 * pi * x) + 0.277263158 * std::cos(4 * pi * x) -
+                        0.083578947 * std::cos(6 * pi * x) + 0.006947368 * std::cos(8 * pi * x);
+                    break;
+                case kaiser: {
+                    const double half = (double)(n - 1) / 2.0, y = ((double)k - half) / half;
+                    v = i0(beta * std::sqrt(std::fmax(0.0, 1.0 - y * y))) / i0(beta);
+                    break;
+                }
+                case rectangular: v = 1.0; break;
+            }
+            w[(size_t)k] = (float)v;
+        }
+        return w;
+    }
+};
+
+// Analyser of `channels` signals at one FFT size (mifft_xspec): the frame
+// list and scratch shared by STFT, Welch and cross-spectral analysis.
+class Analyser {
+   public:
+    Analyser(int64_t fftSize, int64_t channels) : fftSize(fftSize), channels(channels) {
+        FFTError::check(mifft_xspec_create(fftSize, channels, &x_));
+    }
+    ~Analyser() { mifft_xspec_destroy(x_); }
+    Analyser(const Analyser&) = delete;
+    Analyser& operator=(const Analyser&) = delete;
+    const int64_t fftSize, channels;
+    int64_t frames(int64_t samples, int64_t hop) const { return mifft_xspec_frames(x_, samples, hop); }
+    mifft_xspec* handle() { return x_; }
+
+   private:
+    mifft_xspec* x_ = nullptr;
+};
+
+// STFTFrame (STFT.swift:5-15); magnitudesDB as Spectrum.toDecibels computes
+// it (vDSP_vdbcon flag 1: 20 log10, floor -120).
+struct STFTFrame {
+    Signal complex;
+    std::vector<float> magnitudes;
+    std::vector<float> magnitudesDB() const {
+        std::vector<float> d(magnitudes.size());
+        for (size_t i = 0; i < d.size(); ++i) d[i] = std::fmax(20.f * std::log10(magnitudes[i]), -120.f);
+        return d;
+    }
+    std::vector<float> phase() const {
+        std::vector<float> p(complex.size());
+        for (size_t i = 0; i < p.size(); ++i) p[i] = std::atan2(complex[i].imag(), complex[i].real());
+        return p;
+    }
+};
+
+// STFT (STFT.swift:20-99): every frame of the signal in one launch.
+class STFT {
+   public:
+    STFT(int64_t fftSize, int64_t hopSize, WindowType window = WindowType{}, double sampleRate = 44100.0)
+        : fftSize(fftSize), hopSize(hopSize), sampleRate(sampleRate), an_(fftSize, 1),
+          window_(window.coefficients(fftSize)) {
+        if (hopSize < 1) throw FFTError(MIFFT_ERR_PRECONDITION, "hopSize must be >= 1", -1, hopSize, -1);
+    }
+    const int64_t fftSize, hopSize;
+    const double sampleRate;
+    int64_t frameCount(int64_t sampleCount) const { return an_.frames(sampleCount, hopSize); }
+    double binFrequency(int64_t bin) const { return (double)bin * sampleRate / (double)fftSize; }
+    double frameTime(int64_t frameIndex) const { return (double)(frameIndex * hopSize) / sampleRate; }
+
+    std::vector<STFTFrame> analyze(const std::vector<float>& signal) {
+        const int64_t f = frameCount((int64_t)signal.size());
+        std::vector<STFTFrame> out((size_t)f);
+        if (f == 0) return out;
+        Signal spec((size_t)(f * fftSize));
+        std::vector<float> pw((size_t)(f * fftSize));
+        FFTError::check(mifft_stft_host(an_.handle(), signal.data(), (int64_t)signal.size(), hopSize,
+                                        window_.data(), fp(spec.data()), pw.data(), nullptr, 20.f));
+        for (int64_t i = 0; i < f; ++i) {
+            out[(size_t)i].complex.assign(spec.begin() + i * fftSize, spec.begin() + (i + 1) * fftSize);
+            out[(size_t)i].magnitudes.assign(pw.begin() + i * fftSize, pw.begin() + (i + 1) * fftSize);
+        }
+        return out;
+    }
+    // [time][frequency] dB magnitudes (STFT.swift:82-85)
+    std::vector<std::vector<float>> spectrogram(const std::vector<float>& signal) {
+        std::vector<std::vector<float>> out;
+        for (const STFTFrame& fr : analyze(signal)) out.push_back(fr.magnitudesDB());
+        return out;
+    }
+
+   private:
+    Analyser an_;
+    std::vector<float> window_;
+};
+
+// FFTAnalyzer.analyze (FFTAnalyzer.swift:92-116): one windowed frame.
+class FFTAnalyzer {
+   public:
+    FFTAnalyzer(int64_t size, double sampleRate, WindowType window = WindowType{})
+        : size(size), sampleRate(sampleRate), stft_(size, 1, window, sampleRate) {}
+    const int64_t size;
+    const double sampleRate;
+    STFTFrame analyze(const std::vector<float>& samples, int64_t offset = 0) {
+        if (offset == 0 && (int64_t)samples.size() != size)
+            throw FFTError(MIFFT_ERR_INVALID_INPUT_SIZE, "samples.count must equal size", size,
+                           (int64_t)samples.size(), -1);
+        if (offset < 0 || offset + size > (int64_t)samples.size())
+            throw FFTError(MIFFT_ERR_PRECONDITION, "offset + size must be <= samples.count", -1, offset, -1);
+        std::vector<float> frame(samples.begin() + offset, samples.begin() + offset + size);
+        return stft_.analyze(frame).at(0);
+    }
+
+   private:
+    STFT stft_;
+};
+
+// CrossSpectralResult (CrossSpectral.swift:11-39) plus welchMSC, the
+// coherence of the averaged spectra.
+struct CrossSpectralResult {
+    int64_t channels = 0, fftSize = 0;
+    std::vector<std::vector<float>> power, coherence, welchMSC;
+    std::vector<Signal> crossSpectra;
+    int64_t pairCount() const { return channels * (channels - 1) / 2; }
+    int64_t pairIndex(int64_t i, int64_t j) const {
+        if (!(0 <= i && i < j && j < channels))
+            throw FFTError(MIFFT_ERR_PRECONDITION, "pairIndex requires i < j < channels", -1, -1, -1);
+        int64_t p = 0;
+        for (int64_t a = 0; a < i; ++a) p += channels - 1 - a;
+        return p + (j - i - 1);
+    }
+};
+
+// PSD (PSD.swift:6-134, CrossSpectral.swift:55-173), each over all frames in
+// batched launches.
+struct PSD {
+    static std::vector<float> welch(const std::vector<float>& signal, int64_t fftSize, int64_t hopSize,
+                                    double sampleRate, WindowType window = WindowType{}) {
+        if (hopSize < 1) throw FFTError(MIFFT_ERR_PRECONDITION, "hopSize must be >= 1", -1, hopSize, -1);
+        Analyser an(fftSize, 1);
+        const std::vector<float> w = window.coefficients(fftSize);
+        std::vector<float> psd((size_t)fftSize);
+        FFTError::check(mifft_welch_host(an.handle(), signal.data(), (int64_t)signal.size(), hopSize, w.data(),
+                                         sampleRate, psd.data()));
+        return psd;
+    }
+    static CrossSpectralResult crossSpectral(const std::vector<std::vector<float>>& channels, int64_t fftSize,
+                                             int64_t hopSize, double /*sampleRate*/ = 0.0,
+                                             WindowType window = WindowType{}) {
+        const int64_t c = (int64_t)channels.size();
+        if (c < 1 || c > 16)
+            throw FFTError(MIFFT_ERR_PRECONDITION, "crossSpectral requires 1-16 channels", -1, c, -1);
+        const int64_t s = (int64_t)channels[0].size();
+        std::vector<float> flat;
+        for (const auto& ch : channels) {
+            if ((int64_t)ch.size() != s)
+                throw FFTError(MIFFT_ERR_PRECONDITION, "all channels must have the same length", s,
+                               (int64_t)ch.size(), -1);
+            flat.insert(flat.end(), ch.begin(), ch.end());
+        }
+        Analyser an(fftSize, c);
+        const int64_t p = c * (c - 1) / 2, n = fftSize;
+        const std::vector<float> w = window.coefficients(n);
+        std::vector<float> pw((size_t)(c * n)), cr((size_t)(2 * p * n + 1)), co((size_t)(p * n + 1)),
+            ms((size_t)(p * n + 1));
+        FFTError::check(mifft_xspec_host(an.handle(), flat.data(), s, hopSize, w.data(), pw.data(), cr.data(),
+                                         co.data(), ms.data()));
+        CrossSpectralResult r;
+        r.channels = c;
+        r.fftSize = n;
+        for (int64_t i = 0; i < c; ++i) r.power.emplace_back(pw.begin() + i * n, pw.begin() + (i + 1) * n);
+        for (int64_t q = 0; q < p; ++q) {
+            const cfloat* z = reinterpret_cast<const cfloat*>(cr.data()) + q * n;
+            r.crossSpectra.emplace_back(z, z + n);
+            r.coherence.emplace_back(co.begin() + q * n, co.begin() + (q + 1) * n);
+            r.welchMSC.emplace_back(ms.begin() + q * n, ms.begin() + (q + 1) * n);
+        }
+        return r;
+    }
+    // |S_ab|^2 / (S_aa S_bb) per bin, frames over min(len a, len b)
+    static std::vector<float> coherence(const std::vector<float>& a, const std::vector<float>& b, int64_t fftSize,
+                                        int64_t hopSize, double sampleRate, WindowType window = WindowType{}) {
+        const size_t m = a.size() < b.size() ? a.size() : b.size();
+        return crossSpectral({std::vector<float>(a.begin(), a.begin() + (int64_t)m),
+                              std::vector<float>(b.begin(), b.begin() + (int64_t)m)},
+                             fftSize, hopSize, sampleRate, window)
+            .welchMSC.at(0);
+    }
 };
 
 }  // namespace mifft

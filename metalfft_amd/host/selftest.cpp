@@ -160,6 +160,32 @@ int main() {
                 if (r[i] > r[arg]) arg = i;
             report(arg == 37, "correlator_lag", (double)arg);
         }
+        {  // STFT frames vs the direct DFT of the windowed slice; Welch of a constant
+            const int64_t n = 256, hop = 96;
+            mifft::STFT stft(n, hop, mifft::WindowType{}, 8000.0);
+            std::mt19937 g(4);
+            std::uniform_real_distribution<float> u(-1.f, 1.f);
+            std::vector<float> sig(1000);
+            for (auto& v : sig) v = u(g);
+            auto frames = stft.analyze(sig);
+            const auto w = mifft::WindowType{}.coefficients(n);
+            double worst = 0;
+            for (size_t f = 0; f < frames.size(); f += 3) {
+                Signal seg((size_t)n);
+                for (int64_t j = 0; j < n; ++j) seg[(size_t)j] = cfloat(sig[f * hop + (size_t)j] * w[(size_t)j], 0.f);
+                worst = std::fmax(worst, rel_l2(Signal(frames[f].complex), dft(seg)));
+            }
+            report(frames.size() == 8 && worst < 1e-6, "stft_frames_vs_dft", worst);
+            std::vector<float> cst((size_t)(7 * n), 0.6f);
+            auto psd = mifft::PSD::welch(cst, n, hop, 8000.0);
+            const double want0 = 2.0 * 0.36 * n / (3.0 * 8000.0);
+            report(std::fabs(psd[0] - want0) < 1e-5 * want0, "welch_constant_closed_form",
+                   std::fabs(psd[0] - want0) / want0);
+            auto coh = mifft::PSD::coherence(sig, std::vector<float>(sig.size(), 0.f), n, hop, 8000.0);
+            double mx = 0;
+            for (float v : coh) mx = std::fmax(mx, std::fabs(v));
+            report(mx == 0.0, "coherence_zero_signal", mx);
+        }
     } catch (const std::exception& e) {
         std::printf("FAIL exception %s\n", e.what());
         return 2;

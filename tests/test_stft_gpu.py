@@ -142,3 +142,40 @@ def test_coherence_vs_fp64(gpu, oracle):
     assert np.max(np.abs(got - want)) < 1e-4
     assert not gpu.PSD.coherence(a, np.zeros_like(a), n, hop, 44100.0).any()
     assert np.allclose(gpu.PSD.coherence(a, 3 * a, n, hop, 44100.0), 1.0, atol=1e-4)
+
+
+def test_host_entry_points_match_device(gpu, oracle):
+    """mifft_stft_host / mifft_welch_host / mifft_xspec_host (blocking, host
+    buffers) give the device forms' results."""
+    import ctypes
+    from metalfft_amd import _lib
+    L = _lib.load()
+    n, hop, c, s = 512, 200, 3, 6000
+    x = rnd((c, s), 41)
+    w = gpu.window_coefficients("hann", n)
+    fp = lambda a: a.ctypes.data_as(_lib.fptr)  # noqa: E731
+    an = gpu.CrossSpectral(n, c)
+    f = an.frames(s, hop)
+    spec = np.empty((c, f, n), np.complex64)
+    pw = np.empty((c, f, n), np.float32)
+    _lib.check(L.mifft_stft_host(an._x, fp(x), s, hop, fp(w), fp(spec.view(np.float32)), fp(pw), None,
+                                 ctypes.c_float(20.0)))
+    for ch in range(c):
+        X = oracle.f64_stft(x[ch], n, hop, w)
+        assert oracle.rel_l2(spec[ch], X) < 1e-6
+        assert oracle.rel_l2(pw[ch], np.abs(X) ** 2) < 2e-6
+    psd = np.empty((c, n), np.float32)
+    _lib.check(L.mifft_welch_host(an._x, fp(x), s, hop, fp(w), ctypes.c_double(1000.0), fp(psd)))
+    for ch in range(c):
+        assert oracle.rel_l2(psd[ch], oracle.f64_welch(x[ch], n, hop, 1000.0, w)) < 1e-5
+    p = c * (c - 1) // 2
+    power = np.empty((c, n), np.float32)
+    cross = np.empty((p, n), np.complex64)
+    coh = np.empty((p, n), np.float32)
+    msc = np.empty((p, n), np.float32)
+    _lib.check(L.mifft_xspec_host(an._x, fp(x), s, hop, fp(w), fp(power), fp(cross.view(np.float32)),
+                                  fp(coh), fp(msc)))
+    pw64, cr64, co64, ms64 = oracle.f64_cross_spectral(x, n, hop, w)
+    assert oracle.rel_l2(power, pw64) < 1e-5 and oracle.rel_l2(cross, cr64) < 1e-5
+    assert np.max(np.abs(coh - co64)) < 1e-4 and np.max(np.abs(msc - ms64)) < 1e-4
+    an.close()
