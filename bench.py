@@ -376,6 +376,34 @@ def extra_xspec(m, torch, c=8, s=1 << 20, n=1024, hop=512):
             "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
+def extra_rocfft(m, torch, n=4096, batch=4096):
+    """Library line (not the reference): torch.fft.fft, i.e. rocFFT through
+    hipFFT, on the headline workload (N = 4096, batch 4096, complex64, four
+    rotating input sets; torch allocates each output from its cache, and
+    out= would add a copy), beside this library on the same sets."""
+    sets = [(torch.randn((batch, n), dtype=torch.complex64, device="cuda"),
+             torch.empty((batch, n), dtype=torch.complex64, device="cuda")) for _ in range(4)]
+    f = m.MetalFFT(n)
+    k = [0]
+
+    def rocfft():
+        a, _ = sets[k[0] % 4]
+        k[0] += 1
+        torch.fft.fft(a, dim=1)
+
+    def ours():
+        a, b = sets[k[0] % 4]
+        k[0] += 1
+        f.execute(a, b, batch)
+    out = {}
+    for name, fn in (("rocfft_torch_fft", rocfft), ("mifft", ours)):
+        ms = time_fn(fn, 40, 5, torch)
+        out[name] = {"ms": ms, "transforms_per_s": batch / (ms * 1e-3),
+                     "roofline_frac": 16 * n * batch / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out["note"] = "stream launches from Python for both (no graph)"
+    return out
+
+
 def extra_sizes(m, torch):
     """Every supported length at 2^24 complex values per launch (128 MiB in,
     128 MiB out), four rotating sets, the autotuned kernel, graph replay of
@@ -543,6 +571,7 @@ def main():
     extras = {}
     if not a.no_extras:
         for key, fn in (("sizes_128MiB", lambda: extra_sizes(m, torch)),
+                        ("rocfft_4096x4096", lambda: extra_rocfft(m, torch)),
                         ("mfma", lambda: extra_mfma(m, torch, O)),
                         ("ragged_10k", lambda: extra_ragged(m, torch, cpu=not a.no_cpu)),
                         ("convolve_512x65536_k1023",
