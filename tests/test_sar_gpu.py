@@ -182,3 +182,39 @@ def test_sar_fp16_mode_without_rcmc_and_errors(gpu, oracle, scene_4096):
     assert e.value.case == "kernelNotFound"
     with pytest.raises(gpu.FFTError):
         RangeDoppler(RdaParams.defaults(precision=7))
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
+def test_point_targets_pslr_snr(gpu, oracle, precision):
+    """The reference's image-quality checks (RadarMetrics) in every precision
+    mode: range PSLR below -13 dB for every target
+    (precision_comparison.swift:590-598) and SNR above 10 dB (main.swift:213),
+    each target on its predicted pixel.  Targets are 400 range and 300
+    azimuth bins apart: the reference's default five
+    (sar_simulator.swift:226-244) put range neighbours 60 bins apart, inside
+    the metric's +-200-bin sidelobe window, where a neighbour's own peak
+    counts as a 0 dB sidelobe."""
+    import torch
+    from metalfft_amd.sar import RadarMetrics, RdaParams, RangeDoppler, expected_bins
+    prm = RdaParams.defaults(io=1, precision=precision)
+    p = oracle.sar_default_params()
+    px = 299_792_458.0 / (2 * prm.sampling_rate)  # range pixel, m
+    az = prm.velocity / prm.prf                    # azimuth pixel, m
+    r0 = prm.center_range
+    tg = [(r0 - 400 * px, -300 * az, 1.0), (r0, 0.0, 1.0), (r0 + 400 * px, 300 * az, 0.8),
+          (r0 + 800 * px, -600 * az, 1.0)]
+    raw = oracle.sar_simulate(p, [(r, x, complex(a)) for r, x, a in tg])
+    rng = np.random.default_rng(3)
+    raw += 0.1 * (rng.standard_normal(raw.shape) + 1j * rng.standard_normal(raw.shape))
+    src = torch.from_numpy(raw.astype(np.complex64).view(np.float32)).cuda().to(torch.float16)
+    img = torch.empty_like(src)
+    RangeDoppler(prm).process(src, img)
+    torch.cuda.synchronize()
+    im = img.float().cpu().numpy().view(np.complex64)
+    rm = RadarMetrics(im, prm.n_range, prm.n_azimuth)
+    for i, (rt, xt, _a) in enumerate(tg):
+        rb, ab = expected_bins(prm, rt, xt)
+        m = rm.measure_target(i, int(round(rb)), int(round(ab)))
+        print(precision, i, round(m["pslrRange"], 2), round(m["pslrAzimuth"], 2), round(m["snr"], 1))
+        assert m["pslrRange"] < -13.0 and m["snr"] > 10.0
+        assert abs(m["measuredRangeBin"] - rb) <= 1.0 and abs(m["measuredAzimuthBin"] - ab) <= 4.0
