@@ -240,13 +240,13 @@ MIFFT_API mifft_status mifft_correlator_cross_device(const mifft_correlator *c, 
                                                      int64_t count, float *out, void *stream);
 
 /* ---------------------------------------------------------- ragged batches */
-/* Transforms of different lengths in at most three launches (no reference counterpart:
+/* Transforms of different lengths in a few launches (no reference counterpart:
  * MetalFFT.forward(batch:) rejects uneven lengths, MetalFFT.swift:108-114). */
 typedef struct mifft_ragged_plan mifft_ragged_plan;
 
 /* Execution mode bits for mifft_ragged_execute_device. */
-#define MIFFT_RAGGED_PERSISTENT 0 /* one persistent launch per tile class present:
-                                     16384, 8192, and lengths up to 4096 */
+#define MIFFT_RAGGED_PERSISTENT 0 /* one launch per length of 4096 and longer, one
+                                     persistent launch for all shorter lengths */
 #define MIFFT_RAGGED_PER_SIZE 1   /* one launch per distinct length (baseline) */
 #define MIFFT_RAGGED_GRAPH 2      /* capture the launch(es) in a hipGraph once per
                                      (in, out, direction, mode) and replay it */
@@ -261,8 +261,8 @@ MIFFT_API mifft_status mifft_ragged_plan_destroy(mifft_ragged_plan *plan);
 MIFFT_API int64_t mifft_ragged_extent(const mifft_ragged_plan *plan);
 /* Sum of lengths (for byte / flop accounting). */
 MIFFT_API int64_t mifft_ragged_elements(const mifft_ragged_plan *plan);
-/* Kernel launches one execution issues in `mode` (the tile classes present,
- * 1-3, or the distinct lengths). */
+/* Kernel launches one execution issues in `mode` (lengths >= 4096 present
+ * plus one if any is shorter, or the distinct lengths). */
 MIFFT_API int64_t mifft_ragged_launches(const mifft_ragged_plan *plan, int mode);
 /* Forward or inverse of every transform, complex64 device buffers (in == out
  * allowed), asynchronous on `stream`.  Not thread-safe per plan. */

@@ -230,14 +230,13 @@ RAGGED_GRAPH = 2
 
 
 class RaggedFFT:
-    """Transforms of different lengths (64 ... 16384) in at most three
-    launches.
+    """Transforms of different lengths (64 ... 16384) in a few launches.
 
     ``lengths[i]`` complex values of transform i start at ``offsets[i]``
     (default: packed back to back).  ``execute`` runs all of them forward or
-    inverse on device buffers; ``mode`` picks one persistent launch per tile
-    class (16384, 8192, up to 4096; default), one launch per length, and/or
-    hipGraph replay."""
+    inverse on device buffers; ``mode`` picks one launch per length of 4096
+    and longer plus one persistent launch for the shorter lengths (default),
+    one launch per length, and/or hipGraph replay."""
 
     def __init__(self, lengths: Sequence[int], offsets: Sequence[int] | None = None):
         n = len(lengths)

@@ -4,19 +4,17 @@
 // No reference counterpart: MetalFFT.forward(batch:) rejects uneven lengths
 // (Sources/MetalFFT/FFT/MetalFFT.swift:108-114).
 //
-// Work is cut into tiles: one transform of 16384 (1024 threads, 128 KiB of
-// LDS), one of 8192 (512 threads, 64 KiB), and for lengths up to 4096 tiles
-// of 4096 elements (256 threads, 32 KiB): 1 of 4096, 2 of 2048, ..., 64 of 64
-// (the last tile of a length may hold fewer).  Length N uses N/16 threads per
-// transform (radix-16 passes, 16 values per thread), so every tile keeps all
-// its threads and LDS busy whatever its length.  Tiles are sorted by length,
-// largest first; each tile class is one persistent launch whose grid is as
-// many workgroups as fit on the GPU at once (one per CU for 16384, two for
-// 8192, four for the rest), striding over the class's tiles.  Splitting the
-// classes lets small lengths run several workgroups per CU, so one
-// workgroup's loads overlap another's passes.  The same kernels launched
-// once per length over that length's tiles are the one-launch-per-size
-// baseline.  Both forms can be captured in a hipGraph and replayed.
+// Lengths of 4096 and longer run one transform per workgroup, one launch per
+// length (k_ragged_one: the uniform kernel's body at a per-transform
+// offset).  Shorter lengths are cut into tiles of 4096 elements (256 threads,
+// 32 KiB of LDS): 2 transforms of 2048, ..., 64 of 64 (the last tile of a
+// length may hold fewer); length N uses N/16 threads per transform
+// (radix-16 passes, 16 values per thread), so every tile keeps all its
+// threads and LDS busy whatever its length.  Tiles are sorted by length,
+// largest first, and one persistent launch (as many workgroups as are
+// resident) strides over them.  The same kernels launched once per length
+// are the one-launch-per-size baseline.  Both forms can be captured in a
+// hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,8 +41,8 @@ int ragged_cus() {
     return cus;
 }
 
-// Tile elements of length 2^l: 16384, 8192, else 4096.
-constexpr int tile_elems(int l) { return l >= 14 ? 16384 : (l == 13 ? 8192 : 4096); }
+// Tile elements of length 2^l: one transform for 4096 and longer, else 4096.
+constexpr int tile_elems(int l) { return l >= 12 ? (1 << l) : 4096; }
 
 // Twiddle tables of every length back to back: the plan table of N = 2^l
 // (W_N^k, k < N, then the packed bases rows of PassTable<N, 16, 0>) starts at
@@ -131,6 +129,40 @@ k_ragged(const float2* in, float2* out, const float2* __restrict__ tw_all,
     if constexpr (LMAX >= 6 && LMIN <= 6) length_loop<6, INV>(in, out, tw_all, tiles, offs, rr.lo[0], rr.hi[0], tile_begin, tile_end, lds);
 }
 
+// Lengths 4096 ... 16384: one workgroup per transform (the uniform kernel's
+// body, k_stockham with swizzled LDS, non-temporal I/O and packed twiddle
+// rows) at offs[blockIdx.x], a uniform (scalar) load: one launch per length,
+// scheduled by the hardware.  A persistent loop over these tiles ran 25-35 %
+// slower than the same transforms as one-shot workgroups.
+template <int N, bool INV>
+__global__ void __launch_bounds__(N / 16)
+k_ragged_one(const float2* in, float2* out, const float2* __restrict__ tw, const int64_t* __restrict__ offs) {
+    constexpr int E = 16;
+    __shared__ float2 lds[N];
+    const int t = threadIdx.x;
+    const int64_t off = offs[blockIdx.x];
+    float2 v[E];
+    load_column<N, E, INV, true>(v, in + off, t, true);
+    PreBases<N, E> pb;
+    pb.load_packed(tw, t);
+    stockham_regs<N, E, LdsSwz, 0, false, true, false, 2>(v, lds, t, tw, pb.w);
+    store_column<N, E, INV, true>(v, out + off, t);
+}
+
+template <int L>
+hipError_t launch_one(const float2* in, float2* out, const float2* tw_all, const int64_t* offs, int count,
+                      bool inverse, hipStream_t st) {
+    constexpr int N = 1 << L;
+    if (count <= 0) return hipSuccess;
+    if (inverse)
+        hipLaunchKernelGGL((k_ragged_one<N, true>), dim3((unsigned)count), dim3(N / 16), 0, st, in, out,
+                           tw_all + table_off(L), offs);
+    else
+        hipLaunchKernelGGL((k_ragged_one<N, false>), dim3((unsigned)count), dim3(N / 16), 0, st, in, out,
+                           tw_all + table_off(L), offs);
+    return hipGetLastError();
+}
+
 // Launches the class kernel over tiles [tile_begin, tile_end) (all of one
 // class) with as many workgroups as are resident on the device at once.
 template <int TILE, int LMIN, int LMAX>
@@ -169,19 +201,25 @@ hipError_t launch_ragged_twiddles(float2* tw_all, hipStream_t st) {
     return hipSuccess;
 }
 
-// Tiles [tile_begin, tile_end) of one class (16384, 8192 or up to 4096).
-hipError_t launch_ragged(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tiles,
-                         const int64_t* offs, const RaggedRanges& rr, int cls, int tile_begin, int tile_end,
-                         bool inverse, hipStream_t st) {
-    switch (cls) {
-        case 0: return launch_class<16384, 14, 14>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, inverse, st);
-        case 1: return launch_class<8192, 13, 13>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, inverse, st);
-        default: return launch_class<4096, 6, 12>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, inverse, st);
+// Lengths 2^l >= 4096: the `count` transforms whose sorted offsets start at
+// offs (one launch).
+hipError_t launch_ragged_one(int l, const float2* in, float2* out, const float2* tw_all, const int64_t* offs,
+                             int count, bool inverse, hipStream_t st) {
+    switch (l) {
+        case 12: return launch_one<12>(in, out, tw_all, offs, count, inverse, st);
+        case 13: return launch_one<13>(in, out, tw_all, offs, count, inverse, st);
+        case 14: return launch_one<14>(in, out, tw_all, offs, count, inverse, st);
     }
+    return hipErrorInvalidValue;
 }
 
-// Class of length 2^l: 0 (16384), 1 (8192), 2 (up to 4096).
-int class_of(int l) { return l >= 14 ? 0 : (l == 13 ? 1 : 2); }
+// Lengths 64 ... 2048: tiles [tile_begin, tile_end) of 4096 elements, one
+// persistent launch.
+hipError_t launch_ragged_tiles(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tiles,
+                               const int64_t* offs, const RaggedRanges& rr, int tile_begin, int tile_end,
+                               bool inverse, hipStream_t st) {
+    return launch_class<4096, 6, 11>(in, out, tw_all, tiles, offs, rr, tile_begin, tile_end, inverse, st);
+}
 
 }  // namespace mifft
 
@@ -194,22 +232,26 @@ mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool 
     const RaggedTile* tiles = static_cast<const RaggedTile*>(rp->d_tiles.ptr);
     const int64_t* offs = static_cast<const int64_t*>(rp->d_offsets.ptr);
     const float2* tw = static_cast<const float2*>(rp->d_twiddles.ptr);
-    if ((mode & MIFFT_RAGGED_PER_SIZE) == 0) {
-        for (int c = 0; c < 3; ++c) {
-            mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, c, rp->class_begin[c],
-                                                     rp->class_begin[c + 1], inverse, st),
-                                       "ragged fft");
-            if (r != MIFFT_OK) return r;
-        }
-        return MIFFT_OK;
-    }
+    // lengths >= 4096: one launch each (their tiles hold one transform, so
+    // the tile range is the sorted transform range)
+    int small_begin = rp->ntiles;
     for (size_t s = 0; s + 1 < rp->size_tile_begin.size(); ++s) {
-        const int l = rp->size_log2[s];
-        mifft_status r = check_hip(launch_ragged(in, out, tw, tiles, offs, rp->ranges, class_of(l),
-                                                 rp->size_tile_begin[s], rp->size_tile_begin[s + 1], inverse, st),
-                                   "ragged fft (per size)");
+        const int l = rp->size_log2[s], b = rp->size_tile_begin[s], e = rp->size_tile_begin[s + 1];
+        mifft_status r = MIFFT_OK;
+        if (l >= 12)
+            r = check_hip(launch_ragged_one(l, in, out, tw, offs + rp->size_first[s], e - b, inverse, st),
+                          "ragged fft");
+        else if (mode & MIFFT_RAGGED_PER_SIZE)
+            r = check_hip(launch_ragged_tiles(in, out, tw, tiles, offs, rp->ranges, b, e, inverse, st),
+                          "ragged fft (per size)");
+        else if (b < small_begin)
+            small_begin = b;
         if (r != MIFFT_OK) return r;
     }
+    if ((mode & MIFFT_RAGGED_PER_SIZE) == 0 && small_begin < rp->ntiles)
+        return check_hip(launch_ragged_tiles(in, out, tw, tiles, offs, rp->ranges, small_begin, rp->ntiles, inverse,
+                                             st),
+                         "ragged fft");
     return MIFFT_OK;
 }
 
@@ -283,17 +325,11 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
     rp->ntiles = (int)tiles.size();
     rp->size_tile_begin = begins;
     rp->size_log2 = log2s;
-    // class c covers the sorted tiles [class_begin[c], class_begin[c + 1])
-    for (int c = 0; c <= 3; ++c) rp->class_begin[c] = (int)tiles.size();
     for (size_t q = 0; q + 1 < begins.size(); ++q) {
         const int l = log2s[q];
         rp->ranges.lo[l - 6] = begins[q];
         rp->ranges.hi[l - 6] = begins[q + 1];
-    }
-    for (int c = 2; c >= 0; --c) {  // classes in descending length order: 0, 1, 2
-        rp->class_begin[c] = rp->class_begin[c + 1];
-        for (size_t q = 0; q + 1 < begins.size(); ++q)
-            if (class_of(log2s[q]) == c && begins[q] < rp->class_begin[c]) rp->class_begin[c] = begins[q];
+        rp->size_first.push_back(tiles[(size_t)begins[q]].first);
     }
     for (int64_t a = 0; a < count; ++a) rp->elements += lengths[a];
     if (hipStreamCreateWithFlags(&rp->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -342,8 +378,12 @@ int64_t mifft_ragged_launches(const mifft_ragged_plan* rp, int mode) {
     if (rp->ntiles == 0) return 0;
     if (mode & MIFFT_RAGGED_PER_SIZE) return (int64_t)rp->size_tile_begin.size() - 1;
     int64_t n = 0;
-    for (int c = 0; c < 3; ++c) n += rp->class_begin[c + 1] > rp->class_begin[c] ? 1 : 0;
-    return n;
+    bool small = false;
+    for (int l : rp->size_log2) {
+        if (l >= 12) ++n;
+        else small = true;
+    }
+    return n + (small ? 1 : 0);
 }
 
 mifft_status mifft_ragged_execute_device(mifft_ragged_plan* rp, const float* in, float* out,

@@ -28,7 +28,7 @@ struct mifft_ragged_plan {
     std::vector<int> size_tile_begin;  // tile range per length, largest length first
     std::vector<int> size_log2;        // log2 length of each of those ranges
     mifft::RaggedRanges ranges = {};   // the same ranges indexed by log2 length
-    int class_begin[4] = {0, 0, 0, 0};  // tile classes 16384, 8192, <= 4096: [class_begin[c], class_begin[c + 1])
+    std::vector<int64_t> size_first;   // sorted index of each length's first transform
     mifft::DeviceBuffer d_tiles, d_offsets, d_twiddles;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;

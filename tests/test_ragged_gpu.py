@@ -1,5 +1,6 @@
-"""Ragged batches (lengths 64 ... 16384 mixed) in one persistent launch per
-tile class, per size, and replayed from a hipGraph, against FP64 per
+"""Ragged batches (lengths 64 ... 16384 mixed): one launch per length of
+4096 and longer plus one persistent launch for the shorter ones, or one
+launch per size, optionally replayed from a hipGraph, against FP64 per
 transform."""
 import numpy as np
 import pytest
@@ -38,7 +39,7 @@ def test_ragged_small(gpu, oracle, mode):
     x = ragged_data(lengths, 1)
     rp = gpu.RaggedFFT(lengths)
     assert rp.extent == x.size and rp.elements == x.size
-    classes = len({0 if n == 16384 else (1 if n == 8192 else 2) for n in lengths})
+    classes = len({int(n) if n >= 4096 else 0 for n in lengths})  # one launch per length >= 4096
     assert rp.launches(mode) == (classes if mode & 1 == 0 else len(set(lengths.tolist())))
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
@@ -54,13 +55,14 @@ def test_ragged_small(gpu, oracle, mode):
 @pytest.mark.parametrize("lengths", [[64, 4096, 128, 1024] * 40, [8192, 64] * 30, [16384] * 5,
                                      [16384, 8192] * 7, [2048] * 3 + [8192]])
 def test_ragged_tile_classes(gpu, oracle, lengths):
-    """Plans with one, two or three tile classes present, persistent and
-    per size."""
+    """Plans with lengths of 4096 and longer (one launch each), shorter
+    lengths (one persistent launch of 4096-element tiles), or both;
+    persistent and per size."""
     import torch
     lengths = np.array(lengths, np.int64)
     x = ragged_data(lengths, 5)
     rp = gpu.RaggedFFT(lengths)
-    classes = len({0 if n == 16384 else (1 if n == 8192 else 2) for n in lengths})
+    classes = len({int(n) if n >= 4096 else 0 for n in lengths})  # one launch per length >= 4096
     assert rp.launches() == classes
     xd = torch.from_numpy(x).cuda()
     for mode in (0, 1):
