@@ -88,6 +88,32 @@ struct AzimuthFilter {
     }
 };
 
+// The same filter read from a table H[row][k] built once per processor
+// (k_az_table: the phase above in FP64 with an FP64 sincospi, rounded once to
+// FP32): a read in place of an FP64 phase and an FP32 sincos per element.
+struct AzimuthTable {
+    const float2* H;
+    int64_t n;  // Doppler bins per row
+    __device__ __forceinline__ float2 at(int64_t row, int k) const { return H[row * n + k]; }
+};
+
+__global__ void k_az_table(const double* __restrict__ A, const double* __restrict__ D, double B, int64_t half_nr,
+                           double sign, int64_t rows, int64_t n, float2* __restrict__ H) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * n) return;
+    const int64_t row = i / n, k = i % n;
+    const double d = D[k];
+    float2 h = make_float2(0.f, 0.f);
+    if (d >= 0.0) {
+        double ph = A[k] + (double)(row - half_nr) * B * d;
+        ph -= rint(ph);
+        double sn, cs;
+        sincospi(2.0 * ph, &sn, &cs);
+        h = make_float2((float)cs, (float)(sign * sn));
+    }
+    H[i] = h;
+}
+
 template <int N>
 constexpr int rows_e() { return N >= 256 ? 16 : 8; }
 template <int N>
@@ -392,10 +418,21 @@ mifft_status rda_turn(const mifft_rda* r, const void* in, void* out, hipStream_t
     return check_hip(corner_turn(r->p.io, in, out, r->p.n_azimuth, r->p.n_range, st), "corner turn");
 }
 
+template <class F>
+mifft_status rda_azimuth_with(const mifft_rda* r, const F& f, const void* in, void* out, hipStream_t st);
+
 mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStream_t st) {
+    if (r->az_table.ptr)
+        return rda_azimuth_with(r, AzimuthTable{static_cast<const float2*>(r->az_table.ptr), r->p.n_azimuth}, in,
+                                out, st);
     AzimuthFilter f{static_cast<const double*>(r->az_a.ptr), static_cast<const double*>(r->az_d.ptr),
                     r->az_b, r->p.n_range / 2,
                     r->p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0f : 1.0f};
+    return rda_azimuth_with(r, f, in, out, st);
+}
+
+template <class F>
+mifft_status rda_azimuth_with(const mifft_rda* r, const F& f, const void* in, void* out, hipStream_t st) {
     const bool half = r->p.precision != MIFFT_CONV_FP32;
     if (!r->p.rcmc) {
         if (half)
@@ -409,7 +446,7 @@ mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStrea
     // azimuth FFT (scaled: the range-Doppler frame is stored, FP16 must not
     // overflow) into the work frame, RCMC from it into out, then filter +
     // IFFT in place
-    mifft_status s = check_hip(rows_io<AzimuthFilter, kRowsForward>(
+    mifft_status s = check_hip(rows_io<F, kRowsForward>(
                                    r->p.io, (int)r->p.n_azimuth, in, r->work.ptr,
                                    r->az_plan->twiddles, f, r->p.n_range, (float)r->az_scale, st),
                                "azimuth FFT");
@@ -420,10 +457,10 @@ mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStrea
                        "RCMC")) != MIFFT_OK)
         return s;
     if (half)
-        return check_hip(rows_h<AzimuthFilter, kRowsFilterInverse>(r->p.precision, r->p.io, out, out,
+        return check_hip(rows_h<F, kRowsFilterInverse>(r->p.precision, r->p.io, out, out,
                                                                    r->az_plan->twiddles, f, r->p.n_range, 1.0f, st),
                          "azimuth filter + IFFT (FP16)");
-    return check_hip(rows_io<AzimuthFilter, kRowsFilterInverse>(r->p.io, (int)r->p.n_azimuth, out, out,
+    return check_hip(rows_io<F, kRowsFilterInverse>(r->p.io, (int)r->p.n_azimuth, out, out,
                                                                  r->az_plan->twiddles, f, r->p.n_range,
                                                                  1.0f, st),
                      "azimuth filter + IFFT");
@@ -502,6 +539,18 @@ mifft_status rda_setup(mifft_rda* r) {
         (s = check_hip(hipMemcpy(r->az_d.ptr, D.data(), ab, hipMemcpyHostToDevice), "hipMemcpy")) !=
             MIFFT_OK)
         return s;
+    // the filter as a table when it is at most 512 MiB (4096 x 4096: 128 MiB)
+    const int64_t cells = p.n_range * p.n_azimuth;
+    if (cells <= ((int64_t)1 << 26)) {
+        if ((s = r->az_table.reserve(sizeof(float2) * (size_t)cells)) != MIFFT_OK) return s;
+        hipLaunchKernelGGL(k_az_table, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, r->stream,
+                           static_cast<const double*>(r->az_a.ptr), static_cast<const double*>(r->az_d.ptr), r->az_b,
+                           p.n_range / 2, p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0 : 1.0, p.n_range,
+                           p.n_azimuth, static_cast<float2*>(r->az_table.ptr));
+        if ((s = check_hip(hipGetLastError(), "azimuth filter table")) != MIFFT_OK ||
+            (s = check_hip(hipStreamSynchronize(r->stream), "hipStreamSynchronize")) != MIFFT_OK)
+            return s;
+    }
     // RCMC (rda_kernels.metal:103-185): shift per Doppler bin in range bins,
     // dR = R0 (1/sqrt(1 - ratio^2) - 1) at the centre range, 0 where
     // |ratio| >= 0.95; output bin r reads r + shift (matched: migrated energy
@@ -623,6 +672,7 @@ mifft_status mifft_rda_destroy(mifft_rda* r) {
     r->range_filter.release();
     r->az_a.release();
     r->az_d.release();
+    r->az_table.release();
     r->rcmc_si.release();
     r->rcmc_w.release();
     r->work.release();
