@@ -110,8 +110,13 @@ __device__ __forceinline__ void dft(float2* x) {
 // ------------------------------------------------------------ radix schedule
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
-// Stockham schedule for N = 2^L with at most E elements per thread: full
-// radix-E passes first, one smaller radix pass last for the remainder.
+// Stockham schedule for N = 2^L with at most E elements per thread: one
+// smaller radix pass first for the remainder, then full radix-E passes.
+// Pass 0 needs no twiddles, so the small radix there (E/R butterflies per
+// thread) costs none, and every twiddled pass is one radix-E butterfly per
+// thread with log2 E table entries (fewer registers and table rows than
+// with the remainder last: 16384 = 4*16*16*16 has 1092 packed rows, not
+// 4368).
 template <int N, int E>
 struct Schedule {
     static constexpr int L = ilog2c(N);
@@ -120,7 +125,7 @@ struct Schedule {
     static constexpr int rem = L % e;
     static constexpr int passes = full + (rem ? 1 : 0);
     static constexpr int T = N / E;  // threads per transform
-    static constexpr int radix(int p) { return p < full ? E : (1 << rem); }
+    static constexpr int radix(int p) { return (rem && p == 0) ? (1 << rem) : E; }
     static constexpr int stride(int p) { return p == 0 ? 1 : stride(p - 1) * radix(p - 1); }
 };
 
