@@ -6,12 +6,13 @@
 // for batched N=4096 in Sources/MetalFFT/FFT/MetalFFT.swift:163-165); this
 // replaces it with measurement.
 //
-// Cache: text file, first line "mifft-autotune 5 <arch>", then one
+// Cache: text file, first line "mifft-autotune 6 <arch>", then one
 // "<kind> <dtype> <n> <variant-name> <ns-per-transform>" line per entry.
 // Location: $MIFFT_AUTOTUNE_CACHE, else $XDG_CACHE_HOME/mifft/autotune.txt,
 // else $HOME/.cache/mifft/autotune.txt.  A header mismatch (other version or
 // arch) discards the file.  MIFFT_AUTOTUNE=0 skips timing and uses the
-// built-in defaults; MIFFT_AUTOTUNE=force re-times and rewrites entries.
+// built-in defaults; MIFFT_AUTOTUNE=force re-times and rewrites entries;
+// MIFFT_AUTOTUNE_LOG=1 prints every candidate's time to stderr.
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -32,7 +33,7 @@
 namespace mifft {
 
 namespace {
-const char* kHeader = "mifft-autotune 5 gfx950";
+const char* kHeader = "mifft-autotune 6 gfx950";
 
 std::mutex g_tune_mutex;
 bool g_loaded = false;
@@ -133,8 +134,8 @@ __global__ void k_fill_noise(float2* __restrict__ a, int64_t n) {
 // 15 % faster than they run on fresh data.  Candidates are timed round-robin
 // (five rounds of eight launches each, after eight warm-up launches) and
 // each keeps its best round, so a drift of the clocks during tuning does not
-// favour whichever ran first.  A stateless packed-row variant ("_swz_nt_pt")
-// within 3 % of the fastest is kept instead: persistent and DMA-staged
+// favour whichever ran first.  A stateless one-shot variant ("_swz_nt_pt",
+// "_half") within 3 % of the fastest is kept instead: persistent and DMA-staged
 // variants trade a per-launch prologue for overlap, so their time per
 // transform moves more with batch size and launch mode.
 const StockhamVariant* time_variants(int n, hipStream_t stream, const float2* tw, double* best_ns) {
@@ -178,6 +179,9 @@ const StockhamVariant* time_variants(int n, hipStream_t stream, const float2* tw
             t[q] = std::min(t[q], 1e6 * ms / kReps / batch);
         }
     }
+    if (const char* lg = std::getenv("MIFFT_AUTOTUNE_LOG"); lg && *lg && *lg != '0')
+        for (size_t q = 0; q < cand.size(); ++q)
+            std::fprintf(stderr, "mifft autotune n=%d %s %.2f ns/transform\n", n, cand[q]->name, t[q]);
     const StockhamVariant* best = nullptr;
     const StockhamVariant* stateless = nullptr;
     double best_t = 1e30, stateless_t = 1e30;
@@ -186,7 +190,8 @@ const StockhamVariant* time_variants(int n, hipStream_t stream, const float2* tw
             best_t = t[q];
             best = cand[q];
         }
-        if (std::strstr(cand[q]->name, "_swz_nt_pt") && t[q] < stateless_t) {
+        if ((std::strstr(cand[q]->name, "_swz_nt_pt") || std::strstr(cand[q]->name, "_half")) &&
+            t[q] < stateless_t) {
             stateless_t = t[q];
             stateless = cand[q];
         }

@@ -117,15 +117,17 @@ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 // thread with log2 E table entries (fewer registers and table rows than
 // with the remainder last: 16384 = 4*16*16*16 has 1092 packed rows, not
 // 4368).
+// E = 32 keeps radix 16 (two butterflies per thread and pass).
 template <int N, int E>
 struct Schedule {
+    static constexpr int RM = E < 16 ? E : 16;  // largest radix
     static constexpr int L = ilog2c(N);
-    static constexpr int e = ilog2c(E);
+    static constexpr int e = ilog2c(RM);
     static constexpr int full = L / e;
     static constexpr int rem = L % e;
     static constexpr int passes = full + (rem ? 1 : 0);
     static constexpr int T = N / E;  // threads per transform
-    static constexpr int radix(int p) { return (rem && p == 0) ? (1 << rem) : E; }
+    static constexpr int radix(int p) { return (rem && p == 0) ? (1 << rem) : RM; }
     static constexpr int stride(int p) { return p == 0 ? 1 : stride(p - 1) * radix(p - 1); }
 };
 

@@ -5,8 +5,8 @@
 // (Sources/MetalFFT/FFT/MetalFFT.swift:108-114).
 //
 // Lengths of 4096 and longer run one transform per workgroup, one launch per
-// length (k_ragged_one: the uniform kernel's body at a per-transform
-// offset).  Shorter lengths are cut into tiles of 4096 elements (256 threads,
+// length (k_ragged_one, and k_ragged_half for 8192 and 16384: the uniform
+// kernels' bodies at a per-transform offset).  Shorter lengths are cut into tiles of 4096 elements (256 threads,
 // 32 KiB of LDS): 2 transforms of 2048, ..., 64 of 64 (the last tile of a
 // length may hold fewer); length N uses N/16 threads per transform
 // (radix-16 passes, 16 values per thread), so every tile keeps all its
@@ -129,10 +129,10 @@ k_ragged(const float2* in, float2* out, const float2* __restrict__ tw_all,
     if constexpr (LMAX >= 6 && LMIN <= 6) length_loop<6, INV>(in, out, tw_all, tiles, offs, rr.lo[0], rr.hi[0], tile_begin, tile_end, lds);
 }
 
-// Lengths 4096 ... 16384: one workgroup per transform (the uniform kernel's
-// body, k_stockham with swizzled LDS, non-temporal I/O and packed twiddle
-// rows) at offs[blockIdx.x], a uniform (scalar) load: one launch per length,
-// scheduled by the hardware.  A persistent loop over these tiles ran 25-35 %
+// Length 4096: one workgroup per transform (the uniform kernel's body,
+// k_stockham with swizzled LDS, non-temporal I/O and packed twiddle rows) at
+// offs[blockIdx.x], a uniform (scalar) load: one launch per length,
+// scheduled by the hardware; 8192 and 16384 likewise with k_ragged_half.  A persistent loop over these tiles ran 25-35 %
 // slower than the same transforms as one-shot workgroups.
 template <int N, bool INV>
 __global__ void __launch_bounds__(N / 16)
@@ -149,17 +149,37 @@ k_ragged_one(const float2* in, float2* out, const float2* __restrict__ tw, const
     store_column<N, E, INV, true>(v, out + off, t);
 }
 
+// 8192 and 16384: the half-LDS exchange form (32 values per thread, N/2
+// entries of LDS), four and two transforms per CU instead of two and one.
+template <int N, bool INV>
+__global__ void __launch_bounds__(N / 32)
+__attribute__((amdgpu_waves_per_eu(4, 8)))
+k_ragged_half(const float2* in, float2* out, const float2* __restrict__ tw, const int64_t* __restrict__ offs) {
+    __shared__ float2 lds[N / 2];
+    const int64_t off = offs[blockIdx.x];
+    half_transform<N, INV, true>(in + off, out + off, tw, lds, threadIdx.x, true);
+}
+
 template <int L>
 hipError_t launch_one(const float2* in, float2* out, const float2* tw_all, const int64_t* offs, int count,
                       bool inverse, hipStream_t st) {
     constexpr int N = 1 << L;
     if (count <= 0) return hipSuccess;
-    if (inverse)
-        hipLaunchKernelGGL((k_ragged_one<N, true>), dim3((unsigned)count), dim3(N / 16), 0, st, in, out,
-                           tw_all + table_off(L), offs);
-    else
-        hipLaunchKernelGGL((k_ragged_one<N, false>), dim3((unsigned)count), dim3(N / 16), 0, st, in, out,
-                           tw_all + table_off(L), offs);
+    if constexpr (L >= 13) {
+        if (inverse)
+            hipLaunchKernelGGL((k_ragged_half<N, true>), dim3((unsigned)count), dim3(N / 32), 0, st, in, out,
+                               tw_all + table_off(L), offs);
+        else
+            hipLaunchKernelGGL((k_ragged_half<N, false>), dim3((unsigned)count), dim3(N / 32), 0, st, in, out,
+                               tw_all + table_off(L), offs);
+    } else {
+        if (inverse)
+            hipLaunchKernelGGL((k_ragged_one<N, true>), dim3((unsigned)count), dim3(N / 16), 0, st, in, out,
+                               tw_all + table_off(L), offs);
+        else
+            hipLaunchKernelGGL((k_ragged_one<N, false>), dim3((unsigned)count), dim3(N / 16), 0, st, in, out,
+                               tw_all + table_off(L), offs);
+    }
     return hipGetLastError();
 }
 

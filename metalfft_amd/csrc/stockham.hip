@@ -106,6 +106,21 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
     return hipGetLastError();
 }
 
+// Half-LDS exchange form (k_stockham_half): one transform per workgroup.
+template <int N, int WPE>
+static hipError_t launch_half(const float2* in, float2* out, const float2* tw, int64_t batch,
+                              bool inverse, hipStream_t stream) {
+    if (batch <= 0) return hipSuccess;
+    if (batch > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (inverse)
+        hipLaunchKernelGGL((k_stockham_half<N, true, true, WPE>), dim3((unsigned)batch),
+                           dim3(N / 32), 0, stream, in, out, tw, batch);
+    else
+        hipLaunchKernelGGL((k_stockham_half<N, false, true, WPE>), dim3((unsigned)batch),
+                           dim3(N / 32), 0, stream, in, out, tw, batch);
+    return hipGetLastError();
+}
+
 #define MIFFT_V(N, E) \
     {N, E, tpw_for<N, E>(), &launch_stockham<N, E, LdsPad, false, false>, "stockham_r" #E "_" #N}
 #define MIFFT_VS(N, E)                                                                          \
@@ -129,6 +144,9 @@ static hipError_t launch_stockham(const float2* in, float2* out, const float2* t
 #define MIFFT_VP3(N)                                                                            \
     {N, 16, tpw_for<N, 16>(), &launch_stockham<N, 16, LdsSwz, true, true, 3>,                   \
      "stockham_r16_" #N "_stream_p3"}
+// half-LDS exchange, E = 32, at least WPE waves per SIMD
+#define MIFFT_VH(N, WPE)                                                                        \
+    {N, 32, 1, &launch_half<N, WPE>, "stockham_r16_" #N "_half" #WPE}
 #define MIFFT_VD(N, E) \
     {N, E, tpw_for<N, E>(), &launch_dma<N, E>, "stockham_r" #E "_" #N "_dma"}
 
@@ -154,6 +172,8 @@ static const StockhamVariant kVariants[] = {
     MIFFT_VT(2048),    MIFFT_VT(4096),    MIFFT_VT(8192),     MIFFT_VT(16384),
     MIFFT_VPT(1024),   MIFFT_VPT(2048),   MIFFT_VPT(4096),    MIFFT_VPT(8192),
     MIFFT_VP3(16384),
+    MIFFT_VH(2048, 4),  MIFFT_VH(4096, 4), MIFFT_VH(8192, 4),
+    MIFFT_VH(16384, 4),
 };
 #undef MIFFT_V
 #undef MIFFT_VS
@@ -162,6 +182,7 @@ static const StockhamVariant kVariants[] = {
 #undef MIFFT_VT
 #undef MIFFT_VPT
 #undef MIFFT_VP3
+#undef MIFFT_VH
 
 const StockhamVariant* stockham_variants(int* count) {
     *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));

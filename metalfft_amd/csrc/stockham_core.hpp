@@ -546,6 +546,117 @@ k_stockham_dma(const float2* in, float2* out, const float2* __restrict__ tw, int
     }
 }
 
+// ------------------------------------------------- half-LDS exchange form
+// E = 32 values per thread (two radix-16 butterflies per pass) and an LDS
+// buffer of N/2 entries: every exchange goes through LDS in two halves.
+// Butterfly b of a pass that is not the last writes all R outputs into one
+// half of the transform, [0, N/2) iff b < N/(2R): its outputs lie in
+// [(b/s)*R*s, (b/s + 1)*R*s) and R*s divides N/2.  With b = t + i*T that is
+// i < B/2 for every thread, so half 0 is written by butterflies i < B/2 and
+// read back as v[m], m < E/2 (positions t + T*m < N/2), then half 1 likewise.
+// Four barriers per exchange instead of two, for half the LDS: at N = 16384
+// two transforms per CU (64 KiB each) instead of one.
+//
+// Bases rows (PassTable<N, E, 0>) of butterfly i of pass p, into w[4i..4i+3].
+template <int N, int E, int P>
+__device__ __forceinline__ void half_bases(float2 (&w)[E / 16 * 4], const float2* __restrict__ tw,
+                                           int t) {
+    using S = Schedule<N, E>;
+    if constexpr (P < S::passes) {
+        constexpr int SD = S::stride(P);
+        constexpr int off = N + PassTable<N, E, 0>::offset(P);
+        static_assert(S::radix(P) == 16, "twiddled passes are radix 16");
+#pragma unroll
+        for (int i = 0; i < E / 16; ++i) {
+            const int pos = (t + i * S::T) & (SD - 1);
+            load_row<2>(&w[4 * i], tw + off + pos * 4);
+        }
+    }
+}
+
+// Pass P on v (w: bases rows of pass P, loaded during the previous
+// exchange), then the exchange and the following passes.
+template <int N, int E, class L, int P>
+__device__ __forceinline__ void stockham_half(float2 (&v)[E], float2* buf, int t,
+                                              const float2* __restrict__ tw,
+                                              const float2 (&w)[E / 16 * 4]) {
+    using S = Schedule<N, E>;
+    constexpr int R = S::radix(P);
+    constexpr int SD = S::stride(P);
+    constexpr int B = E / R;
+    constexpr int T = S::T;
+    constexpr bool LAST = (P == S::passes - 1);
+    static_assert(E == 32 && B % 2 == 0, "two butterflies per thread and pass at least");
+
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        float2 x[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) x[j] = v[i + B * j];
+        if constexpr (P > 0) {
+            float2 wp[R];
+            powers_from_bases<R>(wp, &w[4 * i]);
+#pragma unroll
+            for (int j = 1; j < R; ++j) x[j] = cmul(x[j], wp[j]);
+        }
+        dft<R>(x);
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[i + B * j] = x[j];
+    }
+
+    if constexpr (!LAST) {
+        float2 wn[E / 16 * 4];
+        half_bases<N, E, P + 1>(wn, tw, t);  // in flight through the exchange
+        float2 u[E];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (P > 0 || h > 0) lds_barrier();  // the other half's reads are done
+#pragma unroll
+            for (int i = h * B / 2; i < (h + 1) * B / 2; ++i) {
+                const int b = t + i * T;
+                const int pos = b & (SD - 1);
+                const int base = (b / SD) * (R * SD) + pos - h * (N / 2);
+#pragma unroll
+                for (int j = 0; j < R; ++j) buf[L::idx(base + j * SD)] = v[i + B * j];
+            }
+            lds_barrier();
+#pragma unroll
+            for (int m = h * E / 2; m < (h + 1) * E / 2; ++m)
+                u[m] = buf[L::idx(t + T * m - h * (N / 2))];
+        }
+        stockham_half<N, E, L, P + 1>(u, buf, t, tw, wn);
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = u[m];
+    }
+}
+
+// One transform src -> dst by the N/32 threads t of a workgroup through
+// lds (N/2 entries, swizzled); the caller's workgroup barriers nothing else.
+template <int N, bool INV, bool NT>
+__device__ __forceinline__ void half_transform(const float2* src, float2* dst,
+                                               const float2* __restrict__ tw, float2* lds, int t,
+                                               bool active) {
+    constexpr int E = 32;
+    float2 v[E];
+    load_column<N, E, INV, NT>(v, src, t, active);
+    float2 w[E / 16 * 4];
+    half_bases<N, E, 1>(w, tw, t);  // pass 1's rows, in flight during pass 0
+    stockham_half<N, E, LdsSwz, 0>(v, lds, t, tw, w);
+    if (active) store_column<N, E, INV, NT>(v, dst, t);
+}
+
+// One transform per workgroup of N/32 threads, LDS N/2 entries.  WPE: waves
+// per SIMD the register allocation must allow (4: two 16384-point or four
+// 8192-point transforms per CU).
+template <int N, bool INV, bool NT, int WPE>
+__global__ void __launch_bounds__(N / 32)
+__attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_stockham_half(const float2* in, float2* out, const float2* __restrict__ tw, int64_t batch) {
+    __shared__ float2 lds[N / 2];
+    const int64_t tr = blockIdx.x;
+    half_transform<N, INV, NT>(in + tr * N, out + tr * N, tw, lds, threadIdx.x, tr < batch);
+}
+
 // Transforms per workgroup: pack small transforms up to 256 threads.
 template <int N, int E>
 constexpr int tpw_for() {
