@@ -11,6 +11,8 @@ roofline, the CPU baseline and the reduced-precision forms.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -49,6 +51,20 @@ def parse():
     return p.parse_args()
 
 
+@contextlib.contextmanager
+def no_gc():
+    """Python's cyclic collector off around a timed launch loop: a
+    collection that falls inside it (after the large allocations of earlier
+    measurements) stalls the host for a millisecond or more between two
+    launches, which the events then count as GPU time."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
+
+
 def time_gpu(fn, steps, warmup, torch):
     """Stream loop: `steps` back-to-back launches from Python, HIP events on
     the launch stream around them."""
@@ -58,13 +74,14 @@ def time_gpu(fn, steps, warmup, torch):
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for i in range(steps):
-        fn(i, None)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for i in range(steps):
+            fn(i, None)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     return e0.elapsed_time(e1) / steps, wall
 
 
@@ -88,12 +105,13 @@ def time_graph(fn, steps, warmup, torch):
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    graph.replay()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        e0.record(stream)
+        graph.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     return e0.elapsed_time(e1) / steps, wall, graph
 
 
@@ -154,11 +172,12 @@ def time_fn(fn, reps, warm, torch):
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        fn()
-    e1.record()
-    torch.cuda.synchronize()
+    with no_gc():
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
 
 

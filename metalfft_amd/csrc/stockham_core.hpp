@@ -27,6 +27,7 @@ namespace mifft {
 // ------------------------------------------------------------- LDS layouts
 // Padded: one spare complex per 16 (N + N/16 entries).
 struct LdsPad {
+    static constexpr bool kXor = false;
     static constexpr int len(int n) { return n + n / 16; }
     static __device__ __forceinline__ int idx(int i) { return i + (i >> 4); }
 };
@@ -35,9 +36,47 @@ struct LdsPad {
 // (lane stride 16 or 256 complex) and contiguous reads then hit 32 distinct
 // 8-byte slots per 32 lanes.
 struct LdsSwz {
+    static constexpr bool kXor = true;
     static constexpr int len(int n) { return n; }
     static __device__ __forceinline__ int idx(int i) { return i ^ ((i >> 4) & 31); }
 };
+
+// Reads v[m] = buf[L::idx(t + T*m)], m < E.  For the swizzle and T >= 32 a
+// power of two with t < T: (t + T*m) >> 4 = (t >> 4) + (T/16)*m without
+// carries between the terms, so idx(t + T*m) = (idx(t) ^ c(m)) + T*m with
+// c(m) = ((T/16)*m) & 31, which repeats with period 512/T (T < 512) or is 0:
+// one or a few base addresses and the T*m terms as immediate offsets,
+// instead of the swizzle arithmetic per element.
+template <int E, int T, class L>
+__device__ __forceinline__ void lds_read_column(float2* v, const float2* buf, int t) {
+    if constexpr (sizeof(L) > 0 && L::kXor && T >= 32 && (T & (T - 1)) == 0) {
+        constexpr int P = T >= 512 ? 1 : 512 / T;
+        const int i0 = L::idx(t);
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = buf[(i0 ^ (((T / 16) * (m % P)) & 31)) + T * m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = buf[L::idx(t + T * m)];
+    }
+}
+
+// Writes the R outputs of a butterfly to buf[L::idx(base + j*SD)], j < R,
+// base = (a multiple of R*SD) + pos, pos < SD.  For the swizzle,
+// idx(base + j*SD) = idx(base) ^ idx(j*SD): the bits j*SD sets are zero in
+// base, and the swizzle's source bits (4-8) of base + j*SD are those of base
+// plus those of j*SD without carries, so each output costs one XOR with a
+// constant instead of the swizzle arithmetic.
+template <int R, int SD, int STRIDE, class L>
+__device__ __forceinline__ void lds_write_butterfly(float2* buf, int base, const float2* y) {
+    if constexpr (L::kXor) {
+        const int i0 = L::idx(base);
+#pragma unroll
+        for (int j = 0; j < R; ++j) buf[i0 ^ L::idx(j * SD)] = y[STRIDE * j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) buf[L::idx(base + j * SD)] = y[STRIDE * j];
+    }
+}
 
 // Workgroup barrier ordering LDS only: global loads issued before it stay in
 // flight (a full __syncthreads also waits for them).
@@ -258,12 +297,10 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
             const int b = t + i * T;
             const int pos = b & (SD - 1);
             const int base = (b / SD) * (R * SD) + pos;
-#pragma unroll
-            for (int j = 0; j < R; ++j) buf[L::idx(base + j * SD)] = v[i + B * j];
+            lds_write_butterfly<R, SD, B, L>(buf, base, &v[i]);
         }
         if constexpr (DSB) ds_barrier(); else lds_barrier();
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = buf[L::idx(t + T * m)];
+        lds_read_column<E, T, L>(v, buf, t);
         stockham_regs<N, E, L, P + 1, SYNC0, TW4, DSB, PRE>(v, buf, t, tw, pre);
     }
 }
@@ -421,7 +458,9 @@ k_stockham(const float2* in, float2* out,
     const bool active = tr < batch;
     float2* buf = lds + local * L::len(N);
     float2 v[E];
-    load_column<N, E, INV, NT>(v, in + tr * N, t, active);
+    // a slot past the batch end transforms the last transform again and
+    // stores nothing: no zero-fill path beside the loads
+    load_column<N, E, INV, NT>(v, in + (active ? tr : batch - 1) * N, t, true);
     if constexpr (TWM == 1 || TWM == 2) {
         PreBases<N, E> pb;
         if constexpr (TWM == 1) pb.load(tw, t); else pb.load_packed(tw, t);
@@ -616,13 +655,10 @@ __device__ __forceinline__ void stockham_half(float2 (&v)[E], float2* buf, int t
                 const int b = t + i * T;
                 const int pos = b & (SD - 1);
                 const int base = (b / SD) * (R * SD) + pos - h * (N / 2);
-#pragma unroll
-                for (int j = 0; j < R; ++j) buf[L::idx(base + j * SD)] = v[i + B * j];
+                lds_write_butterfly<R, SD, B, L>(buf, base, &v[i]);
             }
             lds_barrier();
-#pragma unroll
-            for (int m = h * E / 2; m < (h + 1) * E / 2; ++m)
-                u[m] = buf[L::idx(t + T * m - h * (N / 2))];
+            lds_read_column<E / 2, T, L>(&u[h * E / 2], buf, t);  // positions h*N/2 + t + T*m
         }
         stockham_half<N, E, L, P + 1>(u, buf, t, tw, wn);
 #pragma unroll

@@ -163,6 +163,29 @@ def test_in_place_matches_out_of_place(gpu, plans):
         assert torch.equal(out, y)
 
 
+@pytest.mark.parametrize("n", [8192, 16384])
+def test_in_place_every_variant(gpu, plans, n):
+    """Every kernel form of the long lengths (one-shot, persistent with a
+    prefetched next transform, DMA-staged, split-LDS exchange) gives the
+    same bits in place as out of place, forward and inverse."""
+    import torch
+    fft = plans[n]
+    chosen = fft.kernel_name
+    x = torch.from_numpy(normal_c64((11, n), 3 * n)).cuda()
+    try:
+        for name in gpu.kernel_variants(n):
+            fft.select_kernel(name)
+            for direction in (gpu.FORWARD, gpu.INVERSE):
+                out = torch.empty_like(x)
+                fft.execute(x, out, batch=11, direction=direction)
+                y = x.clone()
+                fft.execute(y, y, batch=11, direction=direction)
+                torch.cuda.synchronize()
+                assert torch.equal(out, y), (name, direction)
+    finally:
+        fft.select_kernel(chosen)
+
+
 def test_error_cases(gpu, plans):
     # MetalFFT.swift:70-72 and :110-114
     with pytest.raises(gpu.FFTError) as ei:
