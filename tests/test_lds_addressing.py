@@ -1,0 +1,102 @@
+"""CPU checks of the index identities the LDS exchange code relies on
+(metalfft_amd/csrc/stockham_core.hpp: LdsSwz, lds_read_column,
+lds_write_butterfly, the split-LDS exchange and the radix schedule), over
+every length, values-per-thread count and pass the kernels instantiate."""
+import pytest
+
+SIZES = [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+
+
+def swz(i):
+    """LdsSwz::idx: flip bits 0-4 by bits 4-8."""
+    return i ^ ((i >> 4) & 31)
+
+
+def schedule(n, e):
+    """Schedule<N, E>: the remainder radix first, then radix min(E, 16)."""
+    rm = min(e, 16)
+    lg, lr = n.bit_length() - 1, rm.bit_length() - 1
+    full, rem = divmod(lg, lr)
+    radix = ([1 << rem] if rem else []) + [rm] * full
+    stride = [1]
+    for r in radix[:-1]:
+        stride.append(stride[-1] * r)
+    return radix, stride
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_schedule_covers_n(n):
+    for e in (4, 8, 16, 32):
+        if e > n:
+            continue
+        radix, stride = schedule(n, e)
+        prod = 1
+        for r in radix:
+            prod *= r
+        assert prod == n
+        assert radix[-1] * stride[-1] == n  # last pass leaves v[m] = X[t + T m]
+        assert all(r <= 16 for r in radix) and all(r == radix[-1] for r in radix[1:])
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_swizzle_is_a_bijection(n):
+    assert sorted(swz(i) for i in range(n)) == list(range(n))
+    if n >= 1024:  # the split-LDS buffers hold N/2 entries
+        assert sorted(swz(i) for i in range(n // 2)) == list(range(n // 2))
+
+
+@pytest.mark.parametrize("t_count", [32, 64, 128, 256, 512, 1024, 2048])
+def test_column_read_identity(t_count):
+    """idx(t + T m) = (idx(t) ^ c(m)) + T m, c(m) = ((T/16) m % P) & 31."""
+    period = 1 if t_count >= 512 else 512 // t_count
+    for t in range(t_count):
+        i0 = swz(t)
+        for m in range(32):
+            c = ((t_count // 16) * (m % period)) & 31
+            assert swz(t + t_count * m) == (i0 ^ c) + t_count * m
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_butterfly_write_identity(n):
+    """idx(base + j s) = idx(base) ^ idx(j s) for every butterfly base of
+    every pass that writes to LDS, and in the split exchange for the bases
+    shifted down by a multiple of N/Q."""
+    for e in (4, 8, 16, 32):
+        if e > n:
+            continue
+        radix, stride = schedule(n, e)
+        for r, s in zip(radix[:-1], stride[:-1]):
+            for b in range(n // r):
+                base = (b // s) * (r * s) + (b % s)
+                i0 = swz(base)
+                for j in range(r):
+                    assert swz(base + j * s) == i0 ^ swz(j * s)
+                if e == 32:
+                    h = 1 if base >= n // 2 else 0
+                    sb = base - h * (n // 2)
+                    for j in range(r):
+                        assert swz(sb + j * s) == swz(sb) ^ swz(j * s)
+
+
+@pytest.mark.parametrize("n", [2048, 4096, 8192, 16384])
+def test_split_exchange_halves(n):
+    """E = 32: butterfly i of every thread writes half i // (B/2), each half
+    exactly once, and half h is read back as v[m], m in [16h, 16h + 16)."""
+    e, t_count = 32, n // 32
+    radix, stride = schedule(n, e)
+    for r, s in zip(radix[:-1], stride[:-1]):
+        bpt = e // r
+        hits = [0] * n
+        for t in range(t_count):
+            for i in range(bpt):
+                b = t + i * t_count
+                base = (b // s) * (r * s) + (b % s)
+                h = i // (bpt // 2)
+                for j in range(r):
+                    p = base + j * s
+                    assert p // (n // 2) == h
+                    hits[p] += 1
+        assert hits == [1] * n
+    for t in range(t_count):
+        for m in range(e):
+            assert (t + t_count * m) // (n // 2) == m // 16
