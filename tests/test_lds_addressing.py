@@ -100,3 +100,21 @@ def test_split_exchange_halves(n):
     for t in range(t_count):
         for m in range(e):
             assert (t + t_count * m) // (n // 2) == m // 16
+
+
+def test_mfma_pass_addresses():
+    """The DFT-as-GEMM kernel's closed-form LDS addresses
+    (metalfft_amd/csrc/mfma_fft.hip) equal midx() of the Stockham positions
+    for every lane (w, c, g), tile t4 and element i."""
+    def midx(i):
+        return i + (i >> 4) + ((i >> 10) << 4)
+    for w in range(4):
+        for t4 in range(4):
+            for c in range(16):
+                for g in range(4):
+                    for i in range(4):
+                        b, j = 16 * (4 * w + t4) + c, 4 * g + i
+                        assert midx(b + 256 * j) == 68 * w + c + 1104 * g + 17 * t4 + 272 * i
+                        assert midx(16 * b + j) == 1104 * w + 17 * c + 4 * g + 272 * t4 + i
+                        assert midx(256 * (b // 16) + b % 16 + 16 * j) == \
+                            1104 * w + c + 68 * g + 272 * t4 + 17 * i
