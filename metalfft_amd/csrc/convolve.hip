@@ -84,6 +84,31 @@ __device__ __forceinline__ float ld_win(const Win& W, int n) {
     return __uint_as_float(__float_as_uint(v) & keep);
 }
 
+// Signal s = item / npair and pair q = item % npair, in 32-bit arithmetic
+// when the whole grid fits (a 64-bit division expands to ~270 instructions,
+// the 32-bit one to a few dozen).
+__device__ __forceinline__ void pair_of(int64_t item, int64_t total, int64_t npair, int64_t& s,
+                                        int64_t& q) {
+    if (total <= 0x7fffffffLL) {
+        const uint32_t i32 = (uint32_t)item, n32 = (uint32_t)npair;
+        const uint32_t s32 = i32 / n32;
+        s = s32;
+        q = i32 - s32 * n32;
+    } else {
+        s = item / npair;
+        q = item % npair;
+    }
+}
+
+// Element n of a float array at a 32-bit byte offset from its uniform base
+// (global_load / global_store with a scalar base and a 32-bit lane offset).
+__device__ __forceinline__ float ld_at(const float* base, uint32_t n) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + n * 4u);
+}
+__device__ __forceinline__ void st_at(float* base, uint32_t n, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + n * 4u) = v;
+}
+
 // --------------------------------------------------------------- forward
 template <int N, int E, int TPW>
 __global__ void __launch_bounds__((N / E) * TPW)
@@ -91,14 +116,16 @@ k_conv_fwd(const float* __restrict__ x, float* __restrict__ y, const float2* __r
            const float2* __restrict__ tw, ConvGeom g) {
     constexpr int T = N / E;
     __shared__ float2 lds[TPW * LdsPad::len(N)];
-    const int local = threadIdx.x / T;
-    const int t = threadIdx.x % T;
+    // one pair per workgroup (TPW == 1): item, signal and pair are uniform,
+    // their 64-bit division scalar
+    const int local = TPW == 1 ? 0 : (int)threadIdx.x / T;
+    const int t = TPW == 1 ? (int)threadIdx.x : (int)threadIdx.x % T;
     float2* buf = lds + local * LdsPad::len(N);
     const int64_t item = (int64_t)blockIdx.x * TPW + local;
     const int64_t total = g.count * g.npair;
     const bool active = item < total;
-    const int64_t s = active ? item / g.npair : 0;
-    const int64_t q = active ? item % g.npair : 0;
+    int64_t s = 0, q = 0;
+    if (active) pair_of(item, total, g.npair, s, q);
     const int64_t b0 = 2 * q, b1 = 2 * q + 1;
     const float* xs = x + s * g.S;
     const int64_t ylen = g.S + g.K - 1;
@@ -108,13 +135,26 @@ k_conv_fwd(const float* __restrict__ x, float* __restrict__ y, const float2* __r
     const bool has1 = b1 < g.nblk;
 
     float2 v[E];
-    const Win W0 = window(xs, g.S, w0, N), W1 = window(xs, g.S, w1, N);  // S >= 1
+    // interior pair (both windows inside the signal, every output in range):
+    // plain loads at 32-bit offsets from a uniform base, no clamp or mask
+    const bool interior = active && has1 && w0 >= 0 && w1 + N <= g.S;
+    if (interior) {
+        const float* p0 = xs + w0;
+        const float* p1 = xs + w1;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-        const int n = t + T * m;
-        const float re = ld_win(W0, n);
-        const float im = ld_win(W1, n);
-        v[m] = make_float2(active ? re : 0.f, active && has1 ? im : 0.f);
+        for (int m = 0; m < E; ++m) {
+            const uint32_t n = (uint32_t)(t + T * m);
+            v[m] = make_float2(ld_at(p0, n), ld_at(p1, n));
+        }
+    } else {
+        const Win W0 = window(xs, g.S, w0, N), W1 = window(xs, g.S, w1, N);  // S >= 1
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int n = t + T * m;
+            const float re = ld_win(W0, n);
+            const float im = ld_win(W1, n);
+            v[m] = make_float2(active ? re : 0.f, active && has1 ? im : 0.f);
+        }
     }
     RegTwiddles<N, E> ct;
     ct.load(tw, t);
@@ -126,6 +166,20 @@ k_conv_fwd(const float* __restrict__ x, float* __restrict__ y, const float2* __r
     if (!active) return;
     constexpr float inv_n = 1.0f / (float)N;
     const int ts = fresh(t);
+    if (interior) {  // outputs j = n - (K - 1) < hop of both blocks, all in range
+        float* q0 = ys + b0 * g.hop;
+        float* q1 = ys + b1 * g.hop;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int n = ts + T * m;
+            if (n >= g.K - 1) {
+                const uint32_t j = (uint32_t)(n - (g.K - 1));
+                st_at(q0, j, v[m].x * inv_n);   // Re of conj(.)/N
+                st_at(q1, j, -v[m].y * inv_n);  // Im of conj(.)/N
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int n = ts + T * m;
