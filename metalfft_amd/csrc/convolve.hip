@@ -228,14 +228,28 @@ k_conv_bwd_x(const float* __restrict__ gy, float* __restrict__ gx, const float2*
     const int t = threadIdx.x % T;
     float2* buf = lds + local * LdsPad::len(N);
     const int64_t item = (int64_t)blockIdx.x * TPW + local;
-    const bool active = item < g.count * npx;
-    const int64_t s = active ? item / npx : 0;
-    const int64_t q = active ? item - s * npx : 0;
+    const int64_t total = g.count * npx;
+    const bool active = item < total;
+    int64_t s = 0, q = 0;
+    if (active) pair_of(item, total, npx, s, q);
     const int64_t ylen = g.S + g.K - 1;
     const int64_t p0 = 2 * q * g.hop, p1 = p0 + g.hop;
     const bool has1 = 2 * q + 1 < nbx;
+    // interior pair: both gradient windows inside gy (p1 + N <= ylen, which
+    // is p1 + hop <= S) and every grad_x output in range
+    const bool interior = active && has1 && p1 + N <= ylen;
     float2 z[E];
-    load_grad_pair<N, E>(z, gy + s * ylen, ylen, p0, p1, active, active && has1, t);
+    if (interior) {
+        const float* g0 = gy + s * ylen + p0;
+        const float* g1 = g0 + g.hop;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const uint32_t n = (uint32_t)(t + T * m);
+            z[m] = make_float2(ld_at(g0, n), ld_at(g1, n));
+        }
+    } else {
+        load_grad_pair<N, E>(z, gy + s * ylen, ylen, p0, p1, active, active && has1, t);
+    }
     RegTwiddles<N, E> ct;
     ct.load(tw, t);
     ct.template fft<false>(z, buf, t, tw);
@@ -246,6 +260,19 @@ k_conv_bwd_x(const float* __restrict__ gy, float* __restrict__ gx, const float2*
     constexpr float inv_n = 1.0f / (float)N;
     float* gxs = gx + s * g.S;
     const int tg = fresh(t);
+    if (interior) {
+        float* q0 = gxs + p0;
+        float* q1 = gxs + p1;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int n = tg + T * m;
+            if (n < g.hop) {
+                st_at(q0, (uint32_t)n, z[m].x * inv_n);
+                st_at(q1, (uint32_t)n, -z[m].y * inv_n);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int n = tg + T * m;
@@ -280,28 +307,55 @@ k_conv_bwd_h(const float* __restrict__ x, const float* __restrict__ gy,
     for (int p = 0; p < ppw; ++p) {  // uniform trip count across the workgroup
         const int64_t item = ((int64_t)blockIdx.x * ppw + p) * TPW + local;
         const bool active = item < total;
-        const int64_t s = active ? item / npx : 0;
-        const int64_t q = active ? item - s * npx : 0;
+        int64_t s = 0, q = 0;
+        if (active) pair_of(item, total, npx, s, q);
         const int64_t p0 = 2 * q * g.hop, p1 = p0 + g.hop;
         const bool a1 = active && 2 * q + 1 < nbx;
+        // interior pair: both gradient windows inside gy and both signal
+        // blocks inside x (p1 + N <= ylen, i.e. p1 + hop <= S)
+        const bool interior = a1 && p1 + N <= ylen;
         // opaque copies of t per iteration: LDS and global index arithmetic
         // is recomputed each time instead of hoisted out of the loop and
         // held live through it
         const int tl = fresh(t);
         float2 z[E], u[E];
-        load_grad_pair<N, E>(z, gy + s * ylen, ylen, p0, p1, active, a1, tl);
+        if (interior) {
+            const float* g0 = gy + s * ylen + p0;
+            const float* g1 = g0 + g.hop;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const uint32_t n = (uint32_t)(tl + T * m);
+                z[m] = make_float2(ld_at(g0, n), ld_at(g1, n));
+            }
+        } else {
+            load_grad_pair<N, E>(z, gy + s * ylen, ylen, p0, p1, active, a1, tl);
+        }
         ct.template fft<true>(z, buf, tl, tw);
         __builtin_amdgcn_sched_barrier(0);  // the signal loads stay below the transform
         {
             const float* xs = x + s * g.S;
-            const Win X0 = window(xs, g.S, p0, g.hop), X1 = window(xs, g.S, p1, g.hop);
             const int tu = fresh(t);
+            if (interior) {  // blocks of hop samples, zero past hop
+                const float* x0 = xs + p0;
+                const float* x1 = xs + p1;
 #pragma unroll
-            for (int m = 0; m < E; ++m) {
-                const int n = tu + T * m;
-                const float ur = ld_win(X0, n);
-                const float ui = ld_win(X1, n);
-                u[m] = make_float2(active ? ur : 0.f, a1 ? ui : 0.f);
+                for (int m = 0; m < E; ++m) {
+                    const int n = tu + T * m;
+                    const uint32_t c = (uint32_t)(n < g.hop ? n : g.hop - 1);
+                    uint32_t keep = n < g.hop ? 0xffffffffu : 0u;
+                    __asm__("" : "+v"(keep));
+                    u[m] = make_float2(__uint_as_float(__float_as_uint(ld_at(x0, c)) & keep),
+                                       __uint_as_float(__float_as_uint(ld_at(x1, c)) & keep));
+                }
+            } else {
+                const Win X0 = window(xs, g.S, p0, g.hop), X1 = window(xs, g.S, p1, g.hop);
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const int n = tu + T * m;
+                    const float ur = ld_win(X0, n);
+                    const float ui = ld_win(X1, n);
+                    u[m] = make_float2(active ? ur : 0.f, a1 ? ui : 0.f);
+                }
             }
         }
         ct.template fft<true>(u, buf, fresh(t), tw);
@@ -374,8 +428,8 @@ k_conv_fwd_h(const float* __restrict__ x, float* __restrict__ y, const float2* _
     __shared__ h2 lds[LdsPad::len(N)];
     const int t = threadIdx.x;
     const int64_t item = blockIdx.x;  // one block pair per workgroup; grid == count * npair
-    const int64_t s = item / g.npair;
-    const int64_t q = item % g.npair;
+    int64_t s, q;
+    pair_of(item, g.count * g.npair, g.npair, s, q);
     const int64_t b0 = 2 * q, b1 = 2 * q + 1;
     const float* xs = x + s * g.S;
     const int64_t ylen = g.S + g.K - 1;
@@ -385,15 +439,28 @@ k_conv_fwd_h(const float* __restrict__ x, float* __restrict__ y, const float2* _
     const bool has1 = b1 < g.nblk;
 
     ConvReg<PM> v[16];
-    const Win W0 = window(xs, g.S, w0, N), W1 = window(xs, g.S, w1, N);  // S >= 1
+    const bool interior = has1 && w0 >= 0 && w1 + N <= g.S;  // as in k_conv_fwd
+    if (interior) {
+        const float* p0 = xs + w0;
+        const float* p1 = xs + w1;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int n = t + T * m;
-        const float re = ld_win(W0, n);
-        const float l1 = ld_win(W1, n);
-        const float im = has1 ? l1 : 0.f;
-        if constexpr (PM == kConvPure) v[m] = to_h2(make_float2(re, im));
-        else v[m] = make_float2(re, im);
+        for (int m = 0; m < 16; ++m) {
+            const uint32_t n = (uint32_t)(t + T * m);
+            const float2 x2 = make_float2(ld_at(p0, n), ld_at(p1, n));
+            if constexpr (PM == kConvPure) v[m] = to_h2(x2);
+            else v[m] = x2;
+        }
+    } else {
+        const Win W0 = window(xs, g.S, w0, N), W1 = window(xs, g.S, w1, N);  // S >= 1
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int n = t + T * m;
+            const float re = ld_win(W0, n);
+            const float l1 = ld_win(W1, n);
+            const float im = has1 ? l1 : 0.f;
+            if constexpr (PM == kConvPure) v[m] = to_h2(make_float2(re, im));
+            else v[m] = make_float2(re, im);
+        }
     }
     PreBases<N, 16> pb;
     pb.load_packed(tw, t);
@@ -423,10 +490,15 @@ k_conv_fwd_h(const float* __restrict__ x, float* __restrict__ y, const float2* _
                 constexpr float inv_n = 1.0f / (float)N;
                 r = make_float2(v[m].x * inv_n, -v[m].y * inv_n);
             }
-            const int64_t o0 = b0 * g.hop + j;
-            if (o0 < ylen) ys[o0] = r.x;
-            const int64_t o1 = b1 * g.hop + j;
-            if (has1 && o1 < ylen) ys[o1] = r.y;
+            if (interior) {
+                st_at(ys + b0 * g.hop, (uint32_t)j, r.x);
+                st_at(ys + b1 * g.hop, (uint32_t)j, r.y);
+            } else {
+                const int64_t o0 = b0 * g.hop + j;
+                if (o0 < ylen) ys[o0] = r.x;
+                const int64_t o1 = b1 * g.hop + j;
+                if (has1 && o1 < ylen) ys[o1] = r.y;
+            }
         }
     }
 }
