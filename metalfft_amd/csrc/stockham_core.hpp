@@ -54,6 +54,11 @@ __device__ __forceinline__ void lds_read_column(float2* v, const float2* buf, in
         const int i0 = L::idx(t);
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[(i0 ^ (((T / 16) * (m % P)) & 31)) + T * m];
+    } else if constexpr (!L::kXor && T % 16 == 0) {
+        // padded layout: idx(t + T*m) = idx(t) + (T + T/16)*m
+        const int i0 = L::idx(t);
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = buf[i0 + (T + T / 16) * m];
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = buf[L::idx(t + T * m)];
@@ -72,6 +77,11 @@ __device__ __forceinline__ void lds_write_butterfly(float2* buf, int base, const
         const int i0 = L::idx(base);
 #pragma unroll
         for (int j = 0; j < R; ++j) buf[i0 ^ L::idx(j * SD)] = y[STRIDE * j];
+    } else if constexpr (!L::kXor && SD % 16 == 0) {
+        // padded layout: idx(base + j*SD) = idx(base) + j*(SD + SD/16)
+        const int i0 = L::idx(base);
+#pragma unroll
+        for (int j = 0; j < R; ++j) buf[i0 + j * (SD + SD / 16)] = y[STRIDE * j];
     } else {
 #pragma unroll
         for (int j = 0; j < R; ++j) buf[L::idx(base + j * SD)] = y[STRIDE * j];
