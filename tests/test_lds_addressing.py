@@ -118,3 +118,27 @@ def test_mfma_pass_addresses():
                         assert midx(16 * b + j) == 1104 * w + 17 * c + 4 * g + 272 * t4 + i
                         assert midx(256 * (b // 16) + b % 16 + 16 * j) == \
                             1104 * w + c + 68 * g + 272 * t4 + 17 * i
+
+
+def test_padded_layout_identities():
+    """LdsPad (one spare entry per 16): idx(t + T m) = idx(t) + (T + T/16) m
+    for T a multiple of 16, and idx(base + j s) = idx(base) + j (s + s/16)
+    for s a multiple of 16, as lds_read_column / lds_write_butterfly use."""
+    def pad(i):
+        return i + (i >> 4)
+    for t_count in (16, 32, 64, 128, 256, 512, 1024):
+        for t in range(t_count):
+            for m in range(16):
+                assert pad(t + t_count * m) == pad(t) + (t_count + t_count // 16) * m
+    for n in SIZES:
+        for e in (8, 16, 32):
+            if e > n:
+                continue
+            radix, stride = schedule(n, e)
+            for r, s in zip(radix[:-1], stride[:-1]):
+                if s % 16:
+                    continue
+                for b in range(n // r):
+                    base = (b // s) * (r * s) + (b % s)
+                    for j in range(r):
+                        assert pad(base + j * s) == pad(base) + j * (s + s // 16)
