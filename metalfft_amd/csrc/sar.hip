@@ -334,6 +334,39 @@ k_corner_turn(const typename SarIo<IO>::T* __restrict__ in, typename SarIo<IO>::
     }
 }
 
+// Complex-FP16 corner turn (4-byte elements) for rows and cols multiples of
+// 64 and 16-byte aligned buffers: 16-byte loads and stores (a 16-lane group
+// moves one 256-byte tile row), the 64 x 64 tile in LDS with element
+// (r, col) at word r*64 + (col ^ 4*((r >> 2) & 15)): the 16-byte chunks a
+// row is written in stay whole, and the column reads of the store phase
+// (element (4q + i, c) for lanes q < 16, four consecutive c per wave) land
+// on 64 distinct banks.
+__global__ void __launch_bounds__(256)
+k_corner_turn_h16(const uint4* __restrict__ in, uint4* __restrict__ out, int64_t rows, int64_t cols) {
+    __shared__ uint32_t tile[64 * 64];
+    const int tid = threadIdx.x;
+    const int q = tid & 15, rr = tid >> 4;
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    const int64_t cols4 = cols >> 2, rows4 = rows >> 2;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int r = rr + 16 * p;
+        const uint4 v = in[(r0 + r) * cols4 + (c0 >> 2) + q];
+        reinterpret_cast<uint4*>(tile)[r * 16 + (q ^ ((r >> 2) & 15))] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int c = rr + 16 * p;  // output row c0 + c, columns r0 + 4q .. + 3
+        uint4 o;
+        o.x = tile[(4 * q + 0) * 64 + (c ^ (4 * q))];
+        o.y = tile[(4 * q + 1) * 64 + (c ^ (4 * q))];
+        o.z = tile[(4 * q + 2) * 64 + (c ^ (4 * q))];
+        o.w = tile[(4 * q + 3) * 64 + (c ^ (4 * q))];
+        out[(c0 + c) * rows4 + (r0 >> 2) + q] = o;
+    }
+}
+
 template <int N, int IO, class F, int MODE>
 hipError_t rows_n(const void* in, void* out, const float2* tw, F f, int64_t rows, float scale,
                   hipStream_t st) {
@@ -386,6 +419,12 @@ hipError_t rcmc(int io, const void* in, void* out, const int* si, const float4* 
 
 hipError_t corner_turn(int io, const void* in, void* out, int64_t rows, int64_t cols, hipStream_t st) {
     dim3 grid((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64));
+    if (io == MIFFT_IO_C32H && rows % 64 == 0 && cols % 64 == 0 &&
+        ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0) {
+        hipLaunchKernelGGL(k_corner_turn_h16, grid, dim3(256), 0, st, static_cast<const uint4*>(in),
+                           static_cast<uint4*>(out), rows, cols);
+        return hipGetLastError();
+    }
     if (io == MIFFT_IO_C32H)
         hipLaunchKernelGGL(k_corner_turn<MIFFT_IO_C32H>, grid, dim3(256), 0, st,
                            static_cast<const h2*>(in), static_cast<h2*>(out), rows, cols);
