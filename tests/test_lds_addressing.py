@@ -142,3 +142,27 @@ def test_padded_layout_identities():
                     base = (b // s) * (r * s) + (b % s)
                     for j in range(r):
                         assert pad(base + j * s) == pad(base) + j * (s + s // 16)
+
+
+def test_corner_turn_tile_swizzle():
+    """Complex-FP16 corner turn tile (metalfft_amd/csrc/sar.hip,
+    k_corner_turn_h16): element (r, c) at word r*64 + (c ^ 4*((r >> 2) & 15))
+    is a bijection, keeps each row's 16-byte chunks whole, and the
+    transposed reads of one wave hit 64 distinct banks."""
+    def word(r, c):
+        return r * 64 + (c ^ (4 * ((r >> 2) & 15)))
+    assert sorted(word(r, c) for r in range(64) for c in range(64)) == list(range(4096))
+    for r in range(64):
+        for q in range(16):
+            for k in range(4):
+                assert word(r, 4 * q + k) == r * 64 + 4 * (q ^ ((r >> 2) & 15)) + k
+    for w in range(4):
+        for p in range(4):
+            for i in range(4):
+                banks = set()
+                for lane in range(64):
+                    tid = 64 * w + lane
+                    q, c = tid & 15, (tid >> 4) + 16 * p
+                    assert word(4 * q + i, c) == (4 * q + i) * 64 + (c ^ (4 * q))
+                    banks.add(word(4 * q + i, c) % 64)
+                assert len(banks) == 64
