@@ -129,7 +129,7 @@ constexpr int rows_threads() { return (N / rows_e<N>()) * tpw_for<N, rows_e<N>()
 // kRcmcRows + 7 input rows held in registers (every input read once per
 // thread instead of once per tap); a workgroup is 64 Doppler columns x 4
 // such runs, so every load is one coalesced row segment.
-constexpr int kRcmcRows = 8;
+constexpr int kRcmcRows = 16;
 
 template <int IO>
 __global__ void __launch_bounds__(256)
