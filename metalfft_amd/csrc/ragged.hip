@@ -91,6 +91,9 @@ __device__ __forceinline__ void length_loop(const float2* in, float2* out, const
                                             const RaggedTile* tiles, const int64_t* offs, int lo, int hi,
                                             int tile_begin, int tile_end, float2* lds) {
     constexpr int T = (1 << L) / 16;
+    // a constant expression: as a plain call the recursive table_off became
+    // a device function call per tile (with a full s_waitcnt on entry)
+    constexpr int kOff = table_off(L);
     const int b = lo > tile_begin ? lo : tile_begin;
     const int e = hi < tile_end ? hi : tile_end;
     const int stride = (int)gridDim.x;
@@ -104,7 +107,7 @@ __device__ __forceinline__ void length_loop(const float2* in, float2* out, const
         const int tn = ti + stride < e ? ti + stride : ti;
         const RaggedTile dn = tiles[tn];
         const int64_t offn = offs[dn.first + (local < dn.count ? local : dn.count - 1)];
-        tile_body<(1 << L), INV>(in, out, tw_all + table_off(L), off, local < d.count, lds);
+        tile_body<(1 << L), INV>(in, out, tw_all + kOff, off, local < d.count, lds);
         d = dn;
         off = offn;
     }
