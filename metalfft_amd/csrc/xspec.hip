@@ -224,8 +224,16 @@ __global__ void k_fill_zero(float* __restrict__ p, int64_t n) {
     if (i < n) p[i] = 0.f;
 }
 
-// Sum of squares of window[n] (null: n), one workgroup, fixed order.
-__global__ void __launch_bounds__(256) k_sumsq(const float* __restrict__ w, int64_t n, float* out) {
+// psd[c][k] = sum of |X[k]|^2 over frames / (frames * window sum of squares
+// * sample rate) (PSD.swift:55-58), k < N, from sums[c][k'] over bins
+// k' <= N/2 (bin k > N/2 is bin N - k).  Every 256-thread block forms the
+// window's sum of squares itself (a strided sum and a fixed tree, the same
+// in every block, N <= 16384 values), so no separate launch precedes this
+// one.
+__global__ void __launch_bounds__(256) k_psd_finish(const float* __restrict__ sums,
+                                                    const float* __restrict__ w, int64_t n,
+                                                    int64_t total, float inv_frames_rate,
+                                                    float* __restrict__ psd) {
     __shared__ float part[256];
     float s = 0.f;
     for (int64_t i = threadIdx.x; i < n; i += 256) s += w ? w[i] * w[i] : 1.f;
@@ -235,18 +243,11 @@ __global__ void __launch_bounds__(256) k_sumsq(const float* __restrict__ w, int6
         if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = part[0];
-}
-
-// psd[c][k] = sum of |X[k]|^2 over frames / (frames * window sum of squares
-// * sample rate) (PSD.swift:55-58), k < N, from sums[c][k'] over bins
-// k' <= N/2 (bin k > N/2 is bin N - k).
-__global__ void k_psd_finish(const float* __restrict__ sums, const float* __restrict__ sumsq, int64_t n,
-                             int64_t total, float inv_frames_rate, float* __restrict__ psd) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float sumsq = part[0];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= total) return;
     const int64_t c = i / n, k = i % n, nb = n / 2 + 1;
-    psd[i] = sums[c * nb + (k < nb ? k : n - k)] * (inv_frames_rate / sumsq[0]);
+    psd[i] = sums[c * nb + (k < nb ? k : n - k)] * (inv_frames_rate / sumsq);
 }
 
 mifft_status zero(float* p, int64_t n, hipStream_t st) {
@@ -403,17 +404,15 @@ mifft_status mifft_welch_device(mifft_xspec* x, const float* signals, int64_t sa
     const int64_t iters = (groups + wpc - 1) / wpc;
     const int64_t lvl1 = (wpc + 63) / 64;
     mifft_status s;
-    if ((s = x->aux.reserve(sizeof(float) * (size_t)((wpc + 2 * lvl1 + 1) * cols + 1))) != MIFFT_OK) return s;
+    if ((s = x->aux.reserve(sizeof(float) * (size_t)((wpc + 2 * lvl1 + 1) * cols))) != MIFFT_OK) return s;
     float* part = static_cast<float*>(x->aux.ptr);
     float* sums = part + (wpc + 2 * lvl1) * cols;
-    float* sumsq = sums + cols;
     const RealPower a{signals, samples, hop, frames, c, wpc, iters, window, part, x->plan->real_tw};
     if ((s = check_hip(real_power((int)n, a, st), "Welch power")) != MIFFT_OK ||
         (s = check_hip(sum_rows(part, wpc, (int)cols, part + wpc * cols, part + (wpc + lvl1) * cols, sums, st),
                        "Welch reduction")) != MIFFT_OK)
         return s;
-    hipLaunchKernelGGL(k_sumsq, dim3(1), dim3(256), 0, st, window, n, sumsq);
-    hipLaunchKernelGGL(k_psd_finish, dim3((unsigned)((c * n + 255) / 256)), dim3(256), 0, st, sums, sumsq, n,
+    hipLaunchKernelGGL(k_psd_finish, dim3((unsigned)((c * n + 255) / 256)), dim3(256), 0, st, sums, window, n,
                        c * n, (float)(1.0 / ((double)frames * sample_rate)), psd);
     return check_hip(hipGetLastError(), "Welch PSD");
 }
