@@ -23,6 +23,14 @@
  * Every entry point returns mifft_status.  On failure, mifft_last_error()
  * gives the case's payload (expected / got / batch index) and
  * mifft_last_error_message() the text of the matching FFTError description.
+ *
+ * Scratch: objects (plans, convolvers, analysers, SAR processors) own their
+ * device scratch and grow it on demand in an eager call.  A grown buffer
+ * never frees the one it replaces before the object is destroyed, so a
+ * hipGraph captured at one size stays valid after eager calls at larger
+ * sizes.  *_device entry points issue no memsets (fill kernels instead) and
+ * can be captured once their scratch has grown to the captured size in an
+ * eager call.
  */
 #ifndef MIFFT_H
 #define MIFFT_H
@@ -104,7 +112,13 @@ MIFFT_API mifft_status mifft_device_name(char *buf, size_t len);
 typedef struct mifft_plan mifft_plan;
 
 /* MetalFFT.init(size:) (MetalFFT.swift:27-51): n in {64, ..., 16384} powers of
- * two, else MIFFT_ERR_UNSUPPORTED_SIZE (FFTDescriptor, MetalContext.swift:48-67). */
+ * two, else MIFFT_ERR_UNSUPPORTED_SIZE (FFTDescriptor, MetalContext.swift:48-67).
+ * Creation allocates the plan's twiddle tables and stream and, for FP32
+ * plans, the first creation of a length in a process times the compiled
+ * kernel variants of that length (about 1 GiB of temporary device memory
+ * for some tens of milliseconds; MIFFT_AUTOTUNE=0 skips it).  The choice is
+ * kept in memory; it is written to disk only when MIFFT_AUTOTUNE_CACHE
+ * names a file. */
 MIFFT_API mifft_status mifft_plan_create(int64_t n, mifft_plan **plan);
 /* Plan with an explicit arithmetic and device element type; the host entry
  * points below accept complex64 plans only.  MFMA precisions and complex-FP16
@@ -213,7 +227,10 @@ MIFFT_API mifft_status mifft_convolver_set_taps_device(mifft_convolver *cv, cons
  * [count][samples + K - 1] writes grad_signal[count][samples] (correlation of
  * grad_out with the taps) and grad_taps[K] (sum over signals of the
  * correlation of grad_out with the signal).  Signal spectra are recomputed,
- * not stored; the tap sum is reduced in a fixed order (deterministic). */
+ * not stored; the tap sum is reduced in a fixed order (deterministic).  The
+ * per-workgroup tap partials live in convolver-owned scratch: backward calls
+ * on one convolver must not overlap in time (on one stream they are
+ * ordered; on two streams the caller orders them with events). */
 MIFFT_API mifft_status mifft_convolver_backward_device(mifft_convolver *cv, const float *signals,
                                                        int64_t samples, int64_t count,
                                                        const float *grad_out, float *grad_signal,

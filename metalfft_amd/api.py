@@ -26,6 +26,8 @@ from . import _lib
 from ._lib import FORWARD, INVERSE, FFTError
 
 SUPPORTED_SIZES = (64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
+# lengths beyond the reference's table, run as two-pass four-step transforms
+FOURSTEP_SIZES = (32768, 65536, 131072, 262144, 524288, 1048576)
 
 
 def _as_c64(x) -> np.ndarray:
@@ -70,18 +72,47 @@ def _stream_handle(stream) -> ctypes.c_void_p:
     return ctypes.c_void_p(int(h))
 
 
-def _dev_ptr(t) -> int:
+# Element types a device buffer may have, by what the entry point reads or
+# writes there: real float32 samples, complex64 (or its float32 re/im view),
+# complex FP16 (or its float16 re/im view).
+F32 = ("torch.float32",)
+C64 = ("torch.complex64", "torch.float32")
+C32H = ("torch.complex32", "torch.float16")
+
+
+def _dev_ptr(t, nbytes: int | None = None, dtypes=None, what: str = "buffer") -> int:
+    """Device address of ``t`` for a *_device entry point.
+
+    For a torch tensor: it must be on the GPU, contiguous, of one of
+    ``dtypes`` and hold at least ``nbytes`` bytes, else FFTError
+    (invalidArgument) is raised before anything is launched.  An integer is
+    taken as a raw device address, unchecked."""
     if hasattr(t, "data_ptr"):
         if not t.is_cuda:
-            raise FFTError(11, "device entry point needs a GPU tensor")
+            raise FFTError(11, f"{what}: device entry point needs a GPU tensor")
         if not t.is_contiguous():
-            raise FFTError(11, "device tensor must be contiguous")
+            raise FFTError(11, f"{what}: device tensor must be contiguous")
+        if dtypes is not None and str(t.dtype) not in dtypes:
+            raise FFTError(11, f"{what}: dtype {t.dtype} where {' or '.join(dtypes)} is needed")
+        if nbytes is not None:
+            have = t.numel() * t.element_size()
+            if have < nbytes:
+                raise FFTError(11, f"{what}: tensor holds {have} bytes, the call needs {nbytes}",
+                               expected=int(nbytes), got=int(have))
         return int(t.data_ptr())
     return int(t)
 
 
+def _count(v, what: str) -> int:
+    v = int(v)
+    if v < 0:
+        raise FFTError(11, f"negative {what}")
+    return v
+
+
 class MetalFFT:
-    """Batched complex FFT of one length (64 ... 16384, powers of two)."""
+    """Batched complex FFT of one length: 64 ... 16384 (one workgroup per
+    transform) or 32768 ... 2^20 (four-step, two launches), powers of two."""
 
     def __init__(self, size: int):
         L = _lib.load()
@@ -165,25 +196,32 @@ class MetalFFT:
             batch = src.numel() // self.size
             if hasattr(src, "dtype") and not src.is_complex():
                 batch = src.numel() // (2 * self.size)
-        _lib.check(_lib.load().mifft_execute_device(self._plan, _dev_ptr(src), _dev_ptr(dst),
-                                                    int(batch), int(direction),
-                                                    _stream_handle(stream)))
+        batch = _count(batch, "batch")
+        need = 8 * self.size * batch
+        _lib.check(_lib.load().mifft_execute_device(
+            self._plan, _dev_ptr(src, need, C64, "src"), _dev_ptr(dst, need, C64, "dst"),
+            batch, int(direction), _stream_handle(stream)))
 
     def forward_real(self, src, dst, batch: int, half: bool = False, stream=None) -> None:
         """Spectra of ``batch`` real float32 signals of length ``size`` on the
         GPU (mifft_forward_real_device): ``dst`` holds ``size`` complex64 bins
         per signal, or ``size // 2 + 1`` with ``half``."""
-        _lib.check(_lib.load().mifft_forward_real_device(self._plan, _dev_ptr(src), _dev_ptr(dst),
-                                                         int(batch), int(bool(half)),
-                                                         _stream_handle(stream)))
+        batch = _count(batch, "batch")
+        bins = self.size // 2 + 1 if half else self.size
+        _lib.check(_lib.load().mifft_forward_real_device(
+            self._plan, _dev_ptr(src, 4 * self.size * batch, F32, "src"),
+            _dev_ptr(dst, 8 * bins * batch, C64, "dst"), batch, int(bool(half)),
+            _stream_handle(stream)))
 
     def inverse_real(self, src, dst, batch: int, half: bool = False, stream=None) -> None:
         """Real signals from Hermitian spectra (mifft_inverse_real_device):
         ``src`` rows of ``size`` (or ``size // 2 + 1`` with ``half``)
         complex64, ``dst`` ``batch`` x ``size`` float32."""
-        _lib.check(_lib.load().mifft_inverse_real_device(self._plan, _dev_ptr(src), int(bool(half)),
-                                                         _dev_ptr(dst), int(batch),
-                                                         _stream_handle(stream)))
+        batch = _count(batch, "batch")
+        bins = self.size // 2 + 1 if half else self.size
+        _lib.check(_lib.load().mifft_inverse_real_device(
+            self._plan, _dev_ptr(src, 8 * bins * batch, C64, "src"), int(bool(half)),
+            _dev_ptr(dst, 4 * self.size * batch, F32, "dst"), batch, _stream_handle(stream)))
 
     def forwardReal(self, signal) -> np.ndarray:
         """Spectrum of one real host signal (all ``size`` bins), the result
@@ -256,9 +294,10 @@ class RaggedFFT:
 
     def execute(self, src, dst, direction: int = FORWARD, mode: int = RAGGED_PERSISTENT,
                 stream=None) -> None:
-        _lib.check(_lib.load().mifft_ragged_execute_device(self._p, _dev_ptr(src), _dev_ptr(dst),
-                                                           int(direction), int(mode),
-                                                           _stream_handle(stream)))
+        need = 8 * self.extent
+        _lib.check(_lib.load().mifft_ragged_execute_device(
+            self._p, _dev_ptr(src, need, C64, "src"), _dev_ptr(dst, need, C64, "dst"),
+            int(direction), int(mode), _stream_handle(stream)))
 
     def close(self) -> None:
         if getattr(self, "_p", None) and self._p.value:
@@ -290,9 +329,12 @@ class PrecisionFFT:
         return _lib.load().mifft_plan_kernel_name(self._plan).decode()
 
     def execute(self, src, dst, batch: int, direction: int = FORWARD, stream=None) -> None:
-        _lib.check(_lib.load().mifft_execute_device(self._plan, _dev_ptr(src), _dev_ptr(dst),
-                                                    int(batch), int(direction),
-                                                    _stream_handle(stream)))
+        batch = _count(batch, "batch")
+        kinds = C32H if self.io == self.C32H else C64
+        need = (4 if self.io == self.C32H else 8) * self.size * batch
+        _lib.check(_lib.load().mifft_execute_device(
+            self._plan, _dev_ptr(src, need, kinds, "src"), _dev_ptr(dst, need, kinds, "dst"),
+            batch, int(direction), _stream_handle(stream)))
 
     def close(self) -> None:
         if getattr(self, "_plan", None) and self._plan.value:

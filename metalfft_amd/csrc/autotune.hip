@@ -6,15 +6,16 @@
 // for batched N=4096 in Sources/MetalFFT/FFT/MetalFFT.swift:163-165); this
 // replaces it with measurement.
 //
-// Cache: text file, first line "mifft-autotune 6 <arch>", then one
-// "<kind> <dtype> <n> <variant-name> <ns-per-transform>" line per entry.
-// Location: $MIFFT_AUTOTUNE_CACHE, else $XDG_CACHE_HOME/mifft/autotune.txt,
-// else $HOME/.cache/mifft/autotune.txt.  A header mismatch (other version or
-// arch) discards the file.  MIFFT_AUTOTUNE=0 skips timing and uses the
-// built-in defaults; MIFFT_AUTOTUNE=force re-times and rewrites entries;
-// MIFFT_AUTOTUNE_LOG=1 prints every candidate's time to stderr.
+// Choices are kept in memory for the life of the process.  A disk cache is
+// opt-in: only when $MIFFT_AUTOTUNE_CACHE names a file is it read and
+// written (text, first line "mifft-autotune 7 <arch>", then one
+// "<kind> <precision> <io> <n> <variant-name> <ns-per-transform>" line per
+// entry, e.g. "stockham fp32 c64 4096 ..."; a
+// header mismatch discards it); without it the library writes nothing.
+// MIFFT_AUTOTUNE=0 skips timing and uses the built-in defaults;
+// MIFFT_AUTOTUNE=force re-times and rewrites entries; MIFFT_AUTOTUNE_LOG=1
+// prints every candidate's time to stderr.
 #include <hip/hip_runtime.h>
-#include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
@@ -22,30 +23,28 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
 
+#include "fourstep.hpp"
 #include "runtime.hpp"
 
 namespace mifft {
 
 namespace {
-const char* kHeader = "mifft-autotune 6 gfx950";
+const char* kHeader = "mifft-autotune 7 gfx950";
 
 std::mutex g_tune_mutex;
 bool g_loaded = false;
 std::map<std::string, std::string> g_table;  // key -> variant name
 
 std::string cache_path() {
-    if (const char* p = std::getenv("MIFFT_AUTOTUNE_CACHE")) return p;
-    std::string base;
-    if (const char* x = std::getenv("XDG_CACHE_HOME")) base = x;
-    else if (const char* h = std::getenv("HOME")) base = std::string(h) + "/.cache";
-    else return "";
-    return base + "/mifft/autotune.txt";
+    const char* p = std::getenv("MIFFT_AUTOTUNE_CACHE");
+    return p ? p : "";
 }
 
 std::string mode() {
@@ -56,25 +55,22 @@ std::string mode() {
 void load_locked() {
     if (g_loaded) return;
     g_loaded = true;
-    std::ifstream f(cache_path());
+    const std::string path = cache_path();
+    if (path.empty()) return;
+    std::ifstream f(path);
     if (!f) return;
     std::string line;
     if (!std::getline(f, line) || line != kHeader) return;
     while (std::getline(f, line)) {
         std::istringstream ss(line);
-        std::string kind, dtype, n, name;
-        if (ss >> kind >> dtype >> n >> name) g_table[kind + " " + dtype + " " + n] = name;
+        std::string kind, prec, io, n, name;
+        if (ss >> kind >> prec >> io >> n >> name) g_table[kind + " " + prec + " " + io + " " + n] = name;
     }
 }
 
 void save_locked(const std::string& key, const std::string& name, double ns) {
     std::string path = cache_path();
     if (path.empty()) return;
-    std::string dir = path.substr(0, path.find_last_of('/'));
-    // mkdir -p of at most two levels ($HOME/.cache/mifft)
-    std::string parent = dir.substr(0, dir.find_last_of('/'));
-    ::mkdir(parent.c_str(), 0755);
-    ::mkdir(dir.c_str(), 0755);
     std::map<std::string, std::string> lines;
     {
         std::ifstream f(path);
@@ -82,8 +78,8 @@ void save_locked(const std::string& key, const std::string& name, double ns) {
         if (f && std::getline(f, line) && line == kHeader) {
             while (std::getline(f, line)) {
                 std::istringstream ss(line);
-                std::string kind, dtype, n;
-                if (ss >> kind >> dtype >> n) lines[kind + " " + dtype + " " + n] = line;
+                std::string kind, prec, io, n;
+                if (ss >> kind >> prec >> io >> n) lines[kind + " " + prec + " " + io + " " + n] = line;
             }
         }
     }
@@ -114,62 +110,86 @@ const StockhamVariant* by_name(const std::string& name) {
     return nullptr;
 }
 
-// Times each candidate on ~32 MiB of data; returns the fastest.
 // Timing input: uniform values in [-1, 1) from an integer hash (data with
 // real bit activity, as transforms see; zeros would flatter every kernel).
-__global__ void k_fill_noise(float2* __restrict__ a, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ float2 noise2(int64_t i) {
     uint32_t h = (uint32_t)i * 2654435761u;
     h ^= h >> 15;
     h *= 0x2c1b3c6du;
     h ^= h >> 12;
-    a[i] = make_float2((float)(h & 0xffff) / 32768.f - 1.f, (float)(h >> 16) / 32768.f - 1.f);
+    return make_float2((float)(h & 0xffff) / 32768.f - 1.f, (float)(h >> 16) / 32768.f - 1.f);
+}
+__global__ void k_fill_noise(float2* __restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = noise2(i);
+}
+typedef _Float16 h2t __attribute__((ext_vector_type(2)));
+__global__ void k_fill_noise_h(h2t* __restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 v = noise2(i);
+    h2t o = {(_Float16)v.x, (_Float16)v.y};
+    a[i] = o;
 }
 
-// Timed as launches of 2^24 / n transforms (128 MiB in, 128 MiB out)
-// rotating over four disjoint input/output sets (1 GiB in all, four times
-// the 256 MiB Infinity Cache), as a pipeline over fresh batches runs: with
-// only two sets, kernels whose loads allocate in that cache were timed up to
-// 15 % faster than they run on fresh data.  Candidates are timed round-robin
-// (five rounds of eight launches each, after eight warm-up launches) and
-// each keeps its best round, so a drift of the clocks during tuning does not
-// favour whichever ran first.  A stateless one-shot variant ("_swz_nt_pt",
-// "_half") within 3 % of the fastest is kept instead: persistent and DMA-staged
-// variants trade a per-launch prologue for overlap, so their time per
-// transform moves more with batch size and launch mode.
-const StockhamVariant* time_variants(int n, hipStream_t stream, const float2* tw, double* best_ns) {
+bool tune_log() {
+    const char* lg = std::getenv("MIFFT_AUTOTUNE_LOG");
+    return lg && *lg && *lg != '0';
+}
+
+// One timing candidate: a launch of `batch` transforms in -> out.
+struct Candidate {
+    const char* name;
+    std::function<hipError_t(const void* in, void* out, int64_t batch)> run;
+    bool prefer;  // stateless one-shot form, kept when within 3 % of the fastest
+};
+
+// Times the candidates of length n (elements of elem_bytes in HBM) and
+// returns the index of the one to use (-1 if none ran).
+// Timed as launches of 2^24 / n transforms (128 MiB in, 128 MiB out for
+// complex64) rotating over four disjoint input/output sets (1 GiB in all,
+// four times the 256 MiB Infinity Cache), as a pipeline over fresh batches
+// runs: with only two sets, kernels whose loads allocate in that cache were
+// timed up to 15 % faster than they run on fresh data.  Candidates are timed
+// round-robin (five rounds of eight launches each, after eight warm-up
+// launches) and each keeps its best round, so a drift of the clocks during
+// tuning does not favour whichever ran first.  A preferred (stateless,
+// one-shot) candidate within 3 % of the fastest is kept instead: persistent
+// and DMA-staged forms trade a per-launch prologue for overlap, so their
+// time per transform moves more with batch size and launch mode.
+int time_candidates(int n, size_t elem_bytes, const std::vector<Candidate>& cand, hipStream_t stream,
+                    const char* what, double* best_ns) {
     constexpr int kSets = 4, kRounds = 5, kReps = 8;
     const int64_t batch = std::max<int64_t>(1, (int64_t(1) << 24) / n);  // per launch
-    float2 *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, sizeof(float2) * n * kSets * batch) != hipSuccess ||
-        hipMalloc(&b, sizeof(float2) * n * kSets * batch) != hipSuccess) {
+    const int64_t set = (int64_t)n * batch, total = set * kSets;
+    char *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, elem_bytes * total) != hipSuccess || hipMalloc(&b, elem_bytes * total) != hipSuccess) {
         (void)hipGetLastError();
         if (a) (void)hipFree(a);
-        return nullptr;
+        return -1;
     }
-    const int64_t total = (int64_t)n * kSets * batch;
-    hipLaunchKernelGGL(k_fill_noise, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, a,
-                       total);
+    const unsigned grid = (unsigned)((total + 255) / 256);
+    if (elem_bytes == 4)
+        hipLaunchKernelGGL(k_fill_noise_h, dim3(grid), dim3(256), 0, stream, reinterpret_cast<h2t*>(a), total);
+    else
+        hipLaunchKernelGGL(k_fill_noise, dim3(grid), dim3(256), 0, stream, reinterpret_cast<float2*>(a), total);
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    int c = 0;
-    const StockhamVariant* all = stockham_variants(&c);
-    const int64_t set = (int64_t)n * batch;
-    auto run = [&](const StockhamVariant* v, int count) {
+    auto run = [&](const Candidate& c, int count) {
         bool good = true;
         for (int r = 0; r < count && good; ++r)
-            good = v->launch(a + (r % kSets) * set, b + (r % kSets) * set, tw, batch, false, stream) ==
+            good = c.run(a + (r % kSets) * set * elem_bytes, b + (r % kSets) * set * elem_bytes, batch) ==
                    hipSuccess;
         return good;
     };
-    std::vector<const StockhamVariant*> cand;
-    for (int i = 0; i < c; ++i)
-        if (all[i].n == n && run(&all[i], kReps)) cand.push_back(&all[i]);
+    std::vector<int> ok;
+    for (size_t q = 0; q < cand.size(); ++q)
+        if (run(cand[q], kReps)) ok.push_back((int)q);
+    (void)hipGetLastError();
     std::vector<double> t(cand.size(), 1e30);
     for (int round = 0; round < kRounds; ++round) {
-        for (size_t q = 0; q < cand.size(); ++q) {
+        for (int q : ok) {
             (void)hipEventRecord(e0, stream);
             run(cand[q], kReps);
             (void)hipEventRecord(e1, stream);
@@ -179,55 +199,86 @@ const StockhamVariant* time_variants(int n, hipStream_t stream, const float2* tw
             t[q] = std::min(t[q], 1e6 * ms / kReps / batch);
         }
     }
-    if (const char* lg = std::getenv("MIFFT_AUTOTUNE_LOG"); lg && *lg && *lg != '0')
-        for (size_t q = 0; q < cand.size(); ++q)
-            std::fprintf(stderr, "mifft autotune n=%d %s %.2f ns/transform\n", n, cand[q]->name, t[q]);
-    const StockhamVariant* best = nullptr;
-    const StockhamVariant* stateless = nullptr;
-    double best_t = 1e30, stateless_t = 1e30;
-    for (size_t q = 0; q < cand.size(); ++q) {
-        if (t[q] < best_t) {
-            best_t = t[q];
-            best = cand[q];
-        }
-        if ((std::strstr(cand[q]->name, "_swz_nt_pt") || std::strstr(cand[q]->name, "_half")) &&
-            t[q] < stateless_t) {
-            stateless_t = t[q];
-            stateless = cand[q];
-        }
+    if (tune_log())
+        for (int q : ok) std::fprintf(stderr, "mifft autotune %s n=%d %s %.2f ns/transform\n", what, n, cand[q].name, t[q]);
+    int best = -1, pref = -1;
+    for (int q : ok) {
+        if (best < 0 || t[q] < t[best]) best = q;
+        if (cand[q].prefer && (pref < 0 || t[q] < t[pref])) pref = q;
     }
-    if (stateless && stateless_t <= 1.03 * best_t) {
-        best = stateless;
-        best_t = stateless_t;
-    }
+    if (pref >= 0 && t[pref] <= 1.03 * t[best]) best = pref;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
     (void)hipGetLastError();
-    *best_ns = best_t;
+    if (best >= 0) *best_ns = t[best];
     return best;
+}
+
+// The table entry for `key`, or the timed choice among `cand` (recorded).
+// Returns the candidate index, or -1 (caller falls back to its default).
+int tuned_choice(const std::string& key, int n, size_t elem_bytes, const std::vector<Candidate>& cand,
+                 hipStream_t stream, const char* what) {
+    const std::string m = mode();
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    load_locked();
+    if (m != "force") {
+        auto it = g_table.find(key);
+        if (it != g_table.end())
+            for (size_t q = 0; q < cand.size(); ++q)
+                if (it->second == cand[q].name) return (int)q;
+    }
+    double ns = 0;
+    const int q = time_candidates(n, elem_bytes, cand, stream, what, &ns);
+    if (q < 0) return -1;
+    g_table[key] = cand[q].name;
+    save_locked(key, cand[q].name, ns);
+    return q;
 }
 }  // namespace
 
 const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* tw) {
-    const std::string m = mode();
-    if (m == "0") return default_variant(n);
-    std::lock_guard<std::mutex> lock(g_tune_mutex);
-    load_locked();
-    const std::string key = "stockham c64 " + std::to_string(n);
-    if (m != "force") {
-        auto it = g_table.find(key);
-        if (it != g_table.end())
-            if (const StockhamVariant* v = by_name(it->second))
-                if (v->n == n) return v;
+    if (mode() == "0") return default_variant(n);
+    int c = 0;
+    const StockhamVariant* all = stockham_variants(&c);
+    std::vector<const StockhamVariant*> vs;
+    std::vector<Candidate> cand;
+    for (int i = 0; i < c; ++i) {
+        if (all[i].n != n) continue;
+        const StockhamVariant* v = &all[i];
+        vs.push_back(v);
+        cand.push_back({v->name,
+                        [=](const void* in, void* out, int64_t batch) {
+                            return v->launch(static_cast<const float2*>(in), static_cast<float2*>(out), tw,
+                                             batch, false, stream);
+                        },
+                        std::strstr(v->name, "_swz_nt_pt") || std::strstr(v->name, "_half")});
     }
-    double ns = 0;
-    const StockhamVariant* v = time_variants(n, stream, tw, &ns);
-    if (!v) return default_variant(n);
-    g_table[key] = v->name;
-    save_locked(key, v->name, ns);
-    return v;
+    const int q = tuned_choice("stockham fp32 c64 " + std::to_string(n), n, 8, cand, stream, "stockham");
+    return q < 0 ? default_variant(n) : vs[q];
+}
+
+const FourStepVariant* choose_fourstep(int n, hipStream_t stream, const float2* tw) {
+    if (mode() == "0") return find_fourstep(n);
+    int c = 0;
+    const FourStepVariant* all = fourstep_variants(&c);
+    std::vector<const FourStepVariant*> vs;
+    std::vector<Candidate> cand;
+    for (int i = 0; i < c; ++i) {
+        if (all[i].n != n) continue;
+        const FourStepVariant* v = &all[i];
+        vs.push_back(v);
+        // out of place: the output buffer holds the intermediate
+        cand.push_back({v->name,
+                        [=](const void* in, void* out, int64_t batch) {
+                            return v->launch(static_cast<const float2*>(in), static_cast<float2*>(out),
+                                             static_cast<float2*>(out), tw, batch, false, stream);
+                        },
+                        true});
+    }
+    const int q = tuned_choice("fourstep fp32 c64 " + std::to_string(n), n, 8, cand, stream, "fourstep");
+    return q < 0 ? find_fourstep(n) : vs[q];
 }
 
 }  // namespace mifft
@@ -244,16 +295,35 @@ int mifft_kernel_variants(int64_t n, const char** names, int cap) {
         if (names && k < cap) names[k] = v[i].name;
         ++k;
     }
+    const mifft::FourStepVariant* f = mifft::fourstep_variants(&c);
+    for (int i = 0; i < c; ++i) {
+        if (f[i].n != n) continue;
+        if (names && k < cap) names[k] = f[i].name;
+        ++k;
+    }
     return k;
 }
 
 mifft_status mifft_plan_select_kernel(mifft_plan* plan, const char* name) {
     if (!plan || !name) return mifft::fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
-    const mifft::StockhamVariant* v = mifft::by_name(name);
-    if (!v || v->n != plan->n)
-        return mifft::fail(MIFFT_ERR_KERNEL_NOT_FOUND, std::string("HIP kernel not found: ") + name);
-    plan->variant = v;
-    return MIFFT_OK;
+    if (plan->precision == MIFFT_PREC_FP32) {
+        if (plan->fourstep) {
+            int c = 0;
+            const mifft::FourStepVariant* f = mifft::fourstep_variants(&c);
+            for (int i = 0; i < c; ++i)
+                if (f[i].n == plan->n && std::strcmp(f[i].name, name) == 0) {
+                    plan->fourstep = &f[i];
+                    return MIFFT_OK;
+                }
+        } else {
+            const mifft::StockhamVariant* v = mifft::by_name(name);
+            if (v && v->n == plan->n) {
+                plan->variant = v;
+                return MIFFT_OK;
+            }
+        }
+    }
+    return mifft::fail(MIFFT_ERR_KERNEL_NOT_FOUND, std::string("HIP kernel not found: ") + name);
 }
 
 }  // extern "C"

@@ -48,7 +48,9 @@ mifft_status mifft_convolver_create(const float* taps, int64_t k, int64_t fft_si
                         ", kernel.count=" + std::to_string(k));
     if (k < 1) return fail(MIFFT_ERR_PRECONDITION, "kernel must have at least one tap");
     mifft_plan* plan = nullptr;
-    mifft_status s = mifft_plan_create(fft_size, &plan);  // unsupportedFFTSize surfaces here
+    mifft_status s = require_supported_size(fft_size);
+    if (s != MIFFT_OK) return s;
+    s = mifft_plan_create(fft_size, &plan);
     if (s != MIFFT_OK) return s;
     mifft_convolver* cv = new mifft_convolver();
     cv->n = fft_size;
@@ -160,7 +162,9 @@ mifft_status mifft_correlator_create(int64_t fft_size, mifft_correlator** out) {
     if (!out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null correlator pointer");
     *out = nullptr;
     mifft_plan* plan = nullptr;
-    mifft_status s = mifft_plan_create(fft_size, &plan);
+    mifft_status s = require_supported_size(fft_size);
+    if (s != MIFFT_OK) return s;
+    s = mifft_plan_create(fft_size, &plan);
     if (s != MIFFT_OK) return s;
     mifft_correlator* c = new mifft_correlator();
     c->n = fft_size;

@@ -230,6 +230,10 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
                           tid, 256);
         int e = 0;
         if (m > 0.f) (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+        // a subnormal maximum (below 2^-126) would give 2^-e = inf and turn
+        // zero entries into NaN: the scale stops at 2^125 (max |x| 2^-e < 1
+        // still holds, the data are just not normalised all the way)
+        if (e < -125) e = -125;
         const float sc = ldexpf(1.f, -e);
 #pragma unroll
         for (int t4 = 0; t4 < 4; ++t4)

@@ -53,23 +53,36 @@ mifft_status check_hip(hipError_t e, const char* what) {
 }
 
 bool supported_size(int64_t n) { return n >= 64 && n <= 16384 && (n & (n - 1)) == 0; }
+bool plan_size(int64_t n) { return supported_size(n) || fourstep_size(n); }
+
+mifft_status require_supported_size(int64_t n) {
+    if (supported_size(n)) return MIFFT_OK;
+    return fail(MIFFT_ERR_UNSUPPORTED_SIZE,
+                "Unsupported FFT size: " + std::to_string(n) +
+                    ". Supported: 64,128,256,512,1024,2048,4096,8192,16384",
+                -1, n);
+}
 
 mifft_status DeviceBuffer::reserve(size_t want) {
     if (want <= bytes) return MIFFT_OK;
-    release();
-    if (want == 0) return MIFFT_OK;
-    hipError_t e = hipMalloc(&ptr, want);
+    const size_t grow = bytes + bytes / 2;
+    const size_t size = want > grow ? want : grow;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, size);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        ptr = nullptr;
         return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
     }
-    bytes = want;
+    if (ptr) retired.push_back(ptr);
+    ptr = p;
+    bytes = size;
     return MIFFT_OK;
 }
 
 void DeviceBuffer::release() {
     if (ptr) (void)hipFree(ptr);
+    for (void* r : retired) (void)hipFree(r);
+    retired.clear();
     ptr = nullptr;
     bytes = 0;
 }
@@ -130,6 +143,19 @@ mifft_status run(const mifft_plan* plan, const float* in, float* out, int64_t ba
         return check_hip(launch_fft_mfma((int)plan->n, plan->precision, plan->io, in, out,
                                          plan->twiddles, batch, dir == MIFFT_INVERSE, 1.0f, stream),
                          "mfma fft");
+    if (plan->fourstep) {
+        // the intermediate goes through the output unless the call is in place
+        float2* y = reinterpret_cast<float2*>(out);
+        if (in == out) {
+            mifft_status s = plan->fs_tmp.reserve(sizeof(float2) * (size_t)plan->n * (size_t)batch);
+            if (s != MIFFT_OK) return s;
+            y = static_cast<float2*>(plan->fs_tmp.ptr);
+        }
+        return check_hip(plan->fourstep->launch(reinterpret_cast<const float2*>(in),
+                                                reinterpret_cast<float2*>(out), y, plan->twiddles,
+                                                batch, dir == MIFFT_INVERSE, stream),
+                         plan->fourstep->name);
+    }
     hipError_t e = plan->variant->launch(reinterpret_cast<const float2*>(in),
                                          reinterpret_cast<float2*>(out), plan->twiddles, batch,
                                          dir == MIFFT_INVERSE, stream);
@@ -202,7 +228,8 @@ const char* mifft_status_string(mifft_status s) {
         case MIFFT_ERR_INVALID_INPUT_SIZE: return "Input size mismatch";
         case MIFFT_ERR_BATCH_INPUT_SIZE: return "Batch element size mismatch";
         case MIFFT_ERR_UNSUPPORTED_SIZE:
-            return "Unsupported FFT size. Supported: 64,128,256,512,1024,2048,4096,8192,16384";
+            return "Unsupported FFT size. Supported: 64,128,256,512,1024,2048,4096,8192,16384 "
+                   "(four-step: 32768 ... 1048576)";
         case MIFFT_ERR_ALLOCATION: return "Buffer allocation failed";
         case MIFFT_ERR_PRECONDITION: return "Precondition failed";
         case MIFFT_ERR_INVALID_ARGUMENT: return "Invalid argument";
@@ -246,10 +273,11 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
     if (io != MIFFT_IO_C64 && io != MIFFT_IO_C32H)
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown io type");
     *out = nullptr;
-    if (!supported_size(n))
+    if (!plan_size(n))
         return fail(MIFFT_ERR_UNSUPPORTED_SIZE,
                     "Unsupported FFT size: " + std::to_string(n) +
-                        ". Supported: 64,128,256,512,1024,2048,4096,8192,16384",
+                        ". Supported: 64,128,256,512,1024,2048,4096,8192,16384 "
+                        "(four-step: 32768 ... 1048576)",
                     -1, n);
     mifft_status s = context_init();
     if (s != MIFFT_OK) return s;
@@ -267,23 +295,41 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
         delete p;
         return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
     }
-    if ((s = check_hip(hipMalloc(&p->twiddles, sizeof(float2) * twiddle_table_len((int)n)),
-                       "hipMalloc")) != MIFFT_OK) {
+    const bool four = fourstep_size(n);
+    const size_t tw_len = four ? fourstep_table_len((int)n) : twiddle_table_len((int)n);
+    if ((s = check_hip(hipMalloc(&p->twiddles, sizeof(float2) * tw_len), "hipMalloc")) != MIFFT_OK) {
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
     }
-    if ((s = check_hip(hipMalloc(&p->real_tw, sizeof(float2) * real_table_len((int)n)), "hipMalloc")) !=
-        MIFFT_OK) {
+    // real-input tables for the single-pass lengths only (the real entry
+    // points report kernelNotFound for four-step plans)
+    if (!four && (s = check_hip(hipMalloc(&p->real_tw, sizeof(float2) * real_table_len((int)n)),
+                                "hipMalloc")) != MIFFT_OK) {
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
     }
-    if ((s = check_hip(launch_twiddles(p->twiddles, (int)n, p->stream), "twiddles")) != MIFFT_OK ||
-        (s = check_hip(launch_real_table(p->real_tw, (int)n, p->stream), "real-input table")) != MIFFT_OK ||
+    if ((s = check_hip(four ? launch_fourstep_twiddles(p->twiddles, (int)n, p->stream)
+                            : launch_twiddles(p->twiddles, (int)n, p->stream),
+                       "twiddles")) != MIFFT_OK ||
+        (!four && (s = check_hip(launch_real_table(p->real_tw, (int)n, p->stream), "real-input table")) !=
+                      MIFFT_OK) ||
         (s = check_hip(hipStreamSynchronize(p->stream), "hipStreamSynchronize")) != MIFFT_OK) {
         mifft_plan_destroy(p);
         return s;
     }
-    p->variant = choose_variant((int)n, p->stream, p->twiddles);
+    if (four) {
+        p->fourstep = choose_fourstep((int)n, p->stream, p->twiddles);
+        if (!p->fourstep) {
+            mifft_plan_destroy(p);
+            return fail(MIFFT_ERR_KERNEL_NOT_FOUND, "HIP kernel not found for size " + std::to_string(n));
+        }
+        *out = p;
+        return MIFFT_OK;
+    }
+    // the Stockham variant serves FP32 plans (and their real-input entry
+    // points); MFMA plans run their own kernel, so only FP32 plans are timed
+    p->variant = precision == MIFFT_PREC_FP32 ? choose_variant((int)n, p->stream, p->twiddles)
+                                              : find_stockham((int)n, 16);
     if (!p->variant) {
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_KERNEL_NOT_FOUND, "HIP kernel not found for size " + std::to_string(n));
@@ -297,6 +343,7 @@ mifft_status mifft_plan_destroy(mifft_plan* plan) {
     if (plan->stream) (void)hipStreamSynchronize(plan->stream);
     plan->in.release();
     plan->out.release();
+    plan->fs_tmp.release();
     if (plan->twiddles) (void)hipFree(plan->twiddles);
     if (plan->real_tw) (void)hipFree(plan->real_tw);
     if (plan->stream) (void)hipStreamDestroy(plan->stream);
@@ -314,6 +361,7 @@ const char* mifft_plan_kernel_name(const mifft_plan* plan) {
          "mfma_bf16split_4096_c32h"}};
     if (!plan) return "";
     if (plan->precision != MIFFT_PREC_FP32) return mfma_names[plan->io][plan->precision];
+    if (plan->fourstep) return plan->fourstep->name;
     return plan->variant ? plan->variant->name : "";
 }
 

@@ -4,8 +4,10 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/mifft.h"
+#include "fourstep.hpp"
 #include "kernels.hpp"
 
 namespace mifft {
@@ -18,15 +20,27 @@ mifft_status ok();
 // MIFFT_ERR_ALLOCATION) with the HIP error text; hipSuccess -> MIFFT_OK.
 mifft_status check_hip(hipError_t e, const char* what);
 
+// Lengths of the single-workgroup kernels (64 ... 16384): every object
+// built on the transform (convolver, correlator, analysers, ragged, SAR).
 bool supported_size(int64_t n);
+// Lengths a plan accepts: those, and the four-step lengths 32768 ... 2^20.
+bool plan_size(int64_t n);
+// MIFFT_OK for supported_size(n), else the unsupportedFFTSize error.
+mifft_status require_supported_size(int64_t n);
 
 // Process-wide device context (thread-safe, idempotent).
 mifft_status context_init();
 
-// Device buffer that grows on demand (host-pointer entry points).
+// Device buffer that grows on demand.  A grown buffer does not free the
+// allocation it replaces: that one moves to `retired` and is freed by
+// release() (the owning object's destroy), because a hipGraph captured
+// before the growth still holds its address and may be replayed after it.
+// Growth is geometric (at least 1.5x), so the retired allocations add up to
+// at most about twice the final size.
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
+    std::vector<void*> retired;
     mifft_status reserve(size_t want);
     void release();
 };
@@ -34,6 +48,7 @@ struct DeviceBuffer {
 // Kernel variant per size: picked by the autotuner (timed on first use and
 // cached on disk) or the built-in default.
 const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* tw);
+const FourStepVariant* choose_fourstep(int n, hipStream_t stream, const float2* tw);
 
 }  // namespace mifft
 
@@ -41,9 +56,14 @@ struct mifft_plan {
     int64_t n = 0;
     int precision = 0;  // mifft_precision
     int io = 0;         // mifft_io
-    const mifft::StockhamVariant* variant = nullptr;
-    float2* twiddles = nullptr;  // W_N^k, k < N, then pass rows (twiddle_table_len)
+    const mifft::StockhamVariant* variant = nullptr;   // n <= 16384
+    const mifft::FourStepVariant* fourstep = nullptr;  // n > 16384
+    float2* twiddles = nullptr;  // W_N^k, k < N, then pass rows (twiddle_table_len);
+                                 // four-step: fourstep_table_len
     float2* real_tw = nullptr;   // real-input transforms (real_table_len)
     hipStream_t stream = nullptr;
     mifft::DeviceBuffer in, out;
+    // four-step intermediate of an in-place call (an out-of-place call uses
+    // the output buffer); grown by eager calls, see DeviceBuffer
+    mutable mifft::DeviceBuffer fs_tmp;
 };

@@ -311,7 +311,9 @@ mifft_status mifft_xspec_create(int64_t fft_size, int64_t channels, mifft_xspec*
     if (channels < 1 || channels > 16)
         return fail(MIFFT_ERR_PRECONDITION, "crossSpectral requires 1-16 channels");
     mifft_plan* plan = nullptr;
-    mifft_status s = mifft_plan_create(fft_size, &plan);  // unsupportedFFTSize surfaces here
+    mifft_status s = require_supported_size(fft_size);
+    if (s != MIFFT_OK) return s;
+    s = mifft_plan_create(fft_size, &plan);
     if (s != MIFFT_OK) return s;
     mifft_xspec* x = new mifft_xspec();
     x->n = fft_size;

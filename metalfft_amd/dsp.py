@@ -28,7 +28,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import FFTError
-from .api import _dev_ptr, _stream_handle
+from .api import C64, F32, _count, _dev_ptr, _stream_handle
 
 
 def _f32(x) -> np.ndarray:
@@ -76,19 +76,25 @@ class Convolver:
         return out
 
     def apply_device(self, signals, out, samples: int, count: int, stream=None) -> None:
+        samples, count = _count(samples, "samples"), _count(count, "count")
+        out_len = samples + self.kernelLen - 1
         _lib.check(_lib.load().mifft_convolver_apply_device(
-            self._cv, _dev_ptr(signals), int(samples), int(count), _dev_ptr(out),
-            _stream_handle(stream)))
+            self._cv, _dev_ptr(signals, 4 * samples * count, F32, "signals"), samples, count,
+            _dev_ptr(out, 4 * out_len * count, F32, "out"), _stream_handle(stream)))
 
     def set_taps_device(self, taps, stream=None) -> None:
-        _lib.check(_lib.load().mifft_convolver_set_taps_device(self._cv, _dev_ptr(taps),
-                                                               _stream_handle(stream)))
+        _lib.check(_lib.load().mifft_convolver_set_taps_device(
+            self._cv, _dev_ptr(taps, 4 * self.kernelLen, F32, "taps"), _stream_handle(stream)))
 
     def backward_device(self, signals, grad_out, grad_signal, grad_taps, samples: int,
                         count: int, stream=None) -> None:
+        samples, count = _count(samples, "samples"), _count(count, "count")
+        out_len = samples + self.kernelLen - 1
         _lib.check(_lib.load().mifft_convolver_backward_device(
-            self._cv, _dev_ptr(signals), int(samples), int(count), _dev_ptr(grad_out),
-            _dev_ptr(grad_signal), _dev_ptr(grad_taps), _stream_handle(stream)))
+            self._cv, _dev_ptr(signals, 4 * samples * count, F32, "signals"), samples, count,
+            _dev_ptr(grad_out, 4 * out_len * count, F32, "grad_out"),
+            _dev_ptr(grad_signal, 4 * samples * count, F32, "grad_signal"),
+            _dev_ptr(grad_taps, 4 * self.kernelLen, F32, "grad_taps"), _stream_handle(stream)))
 
     def close(self) -> None:
         if getattr(self, "_cv", None) and self._cv.value:
@@ -134,9 +140,11 @@ class Correlator:
         return out
 
     def cross_device(self, a, na: int, b, nb: int, count: int, out, stream=None) -> None:
+        na, nb, count = _count(na, "na"), _count(nb, "nb"), _count(count, "count")
         _lib.check(_lib.load().mifft_correlator_cross_device(
-            self._c, _dev_ptr(a), int(na), _dev_ptr(b), int(nb), int(count), _dev_ptr(out),
-            _stream_handle(stream)))
+            self._c, _dev_ptr(a, 4 * na * count, F32, "a"), na,
+            _dev_ptr(b, 4 * nb * count, F32, "b"), nb, count,
+            _dev_ptr(out, 4 * self.fftSize * count, F32, "out"), _stream_handle(stream)))
 
     def close(self) -> None:
         if getattr(self, "_c", None) and self._c.value:
@@ -236,35 +244,45 @@ class CrossSpectral:
     def frames(self, samples: int, hop: int) -> int:
         return int(_lib.load().mifft_xspec_frames(self._x, int(samples), int(hop)))
 
+    def _opt(self, t, nbytes, kinds, what):
+        return _dev_ptr(t, nbytes, kinds, what) if t is not None else None
+
+    def _inputs(self, signals, samples, window):
+        samples = _count(samples, "samples")
+        n, c = self.fftSize, self.channels
+        return (samples, self._opt(signals, 4 * c * samples, F32, "signals"),
+                self._opt(window, 4 * n, F32, "window"))
+
     def compute_device(self, signals, samples: int, hop: int, window, power, cross, coherence,
                        welch_msc=None, stream=None) -> None:
+        samples, sig, win = self._inputs(signals, samples, window)
+        n, c, p = self.fftSize, self.channels, self.pairs
         _lib.check(_lib.load().mifft_xspec_device(
-            self._x, _dev_ptr(signals) if signals is not None else None, int(samples), int(hop),
-            _dev_ptr(window) if window is not None else None, _dev_ptr(power),
-            _dev_ptr(cross) if cross is not None else None,
-            _dev_ptr(coherence) if coherence is not None else None,
-            _dev_ptr(welch_msc) if welch_msc is not None else None, _stream_handle(stream)))
+            self._x, sig, samples, int(hop), win, self._opt(power, 4 * c * n, F32, "power"),
+            self._opt(cross, 8 * p * n, C64, "cross"),
+            self._opt(coherence, 4 * p * n, F32, "coherence"),
+            self._opt(welch_msc, 4 * p * n, F32, "welch_msc"), _stream_handle(stream)))
 
     def stft_device(self, signals, samples: int, hop: int, window, spectra, power=None,
                     power_db=None, db_alpha: float = 20.0, stream=None) -> None:
         """All frames of all channels windowed and transformed in one launch
         (mifft_stft_device): spectra (C, frames, N) complex64, optional
         power |X|^2 and power_db (C, frames, N) float32."""
+        samples, sig, win = self._inputs(signals, samples, window)
+        cells = self.channels * self.frames(samples, hop) * self.fftSize
         _lib.check(_lib.load().mifft_stft_device(
-            self._x, _dev_ptr(signals) if signals is not None else None, int(samples), int(hop),
-            _dev_ptr(window) if window is not None else None,
-            _dev_ptr(spectra) if spectra is not None else None,
-            _dev_ptr(power) if power is not None else None,
-            _dev_ptr(power_db) if power_db is not None else None, float(db_alpha),
+            self._x, sig, samples, int(hop), win, self._opt(spectra, 8 * cells, C64, "spectra"),
+            self._opt(power, 4 * cells, F32, "power"),
+            self._opt(power_db, 4 * cells, F32, "power_db"), float(db_alpha),
             _stream_handle(stream)))
 
     def welch_device(self, signals, samples: int, hop: int, window, sample_rate: float, psd,
                      stream=None) -> None:
         """PSD.welch of every channel (mifft_welch_device): psd (C, N)."""
+        samples, sig, win = self._inputs(signals, samples, window)
         _lib.check(_lib.load().mifft_welch_device(
-            self._x, _dev_ptr(signals) if signals is not None else None, int(samples), int(hop),
-            _dev_ptr(window) if window is not None else None, float(sample_rate),
-            _dev_ptr(psd) if psd is not None else None, _stream_handle(stream)))
+            self._x, sig, samples, int(hop), win, float(sample_rate),
+            self._opt(psd, 4 * self.channels * self.fftSize, F32, "psd"), _stream_handle(stream)))
 
     def close(self) -> None:
         if getattr(self, "_x", None) and self._x.value:

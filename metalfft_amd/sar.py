@@ -10,7 +10,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _lib
-from .api import _dev_ptr, _stream_handle
+from .api import C32H, C64, _dev_ptr, _stream_handle
 
 __all__ = ["RdaParams", "RangeDoppler", "RadarMetrics", "default_targets", "expected_bins",
            "MATCHED", "REFERENCE_FORMULA", "IO_C64", "IO_C32H"]
@@ -52,22 +52,29 @@ class RangeDoppler:
                                                 ctypes.byref(ns)))
         self.range_scale, self.azimuth_scale, self.chirp_samples = rs.value, az.value, ns.value
 
+    def _frame(self, t, what):
+        """Device address of one n_azimuth x n_range frame in the plan's
+        element type (complex FP16 or complex64), size- and type-checked."""
+        p = self.params
+        c32h = p.io == IO_C32H
+        nbytes = (4 if c32h else 8) * p.n_range * p.n_azimuth
+        return _dev_ptr(t, nbytes, C32H if c32h else C64, what)
+
     def process(self, raw, image, stream=None) -> None:
-        _lib.check(_lib.load().mifft_rda_process_device(self._r, _dev_ptr(raw), _dev_ptr(image),
-                                                        _stream_handle(stream)))
+        _lib.check(_lib.load().mifft_rda_process_device(
+            self._r, self._frame(raw, "raw"), self._frame(image, "image"), _stream_handle(stream)))
 
     def range_compress(self, raw, out, stream=None) -> None:
-        _lib.check(_lib.load().mifft_rda_range_compress_device(self._r, _dev_ptr(raw),
-                                                               _dev_ptr(out), _stream_handle(stream)))
+        _lib.check(_lib.load().mifft_rda_range_compress_device(
+            self._r, self._frame(raw, "raw"), self._frame(out, "out"), _stream_handle(stream)))
 
     def corner_turn(self, src, dst, stream=None) -> None:
-        _lib.check(_lib.load().mifft_rda_corner_turn_device(self._r, _dev_ptr(src), _dev_ptr(dst),
-                                                            _stream_handle(stream)))
+        _lib.check(_lib.load().mifft_rda_corner_turn_device(
+            self._r, self._frame(src, "src"), self._frame(dst, "dst"), _stream_handle(stream)))
 
     def azimuth_compress(self, src, dst, stream=None) -> None:
-        _lib.check(_lib.load().mifft_rda_azimuth_compress_device(self._r, _dev_ptr(src),
-                                                                 _dev_ptr(dst),
-                                                                 _stream_handle(stream)))
+        _lib.check(_lib.load().mifft_rda_azimuth_compress_device(
+            self._r, self._frame(src, "src"), self._frame(dst, "dst"), _stream_handle(stream)))
 
     def close(self) -> None:
         if getattr(self, "_r", None) and self._r.value:

@@ -194,12 +194,22 @@ static void fourstep(int n, int n1, int n2, const cf *in, cf *out, cf *ta, cf *t
 }
 
 static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
-static int supported(int n) { return n >= 64 && n <= 16384 && (n & (n - 1)) == 0; }
+/* The reference's sizes (64 ... 16384) and, beyond them, the four-step
+ * lengths 32768 ... 2^20 of the GPU library. */
+static int supported(int n) { return n >= 64 && n <= (1 << 20) && (n & (n - 1)) == 0; }
 
-/* Size table of Sources/MetalFFT/Internal/MetalContext.swift:48-67. */
+/* Size table of Sources/MetalFFT/Internal/MetalContext.swift:48-67 for 8192
+ * and 16384.  The reference stops at 16384; longer lengths keep its
+ * procedure with n1 = 2^floor(L/2) <= n2 = 2^ceil(L/2) (n = 2^L), the same
+ * orientation as its 8192 = 64 x 128 entry. */
 static void fourstep_split(int n, int *n1, int *n2) {
     if (n == 8192) { *n1 = 64; *n2 = 128; }
-    else { *n1 = 128; *n2 = 128; }
+    else if (n == 16384) { *n1 = 128; *n2 = 128; }
+    else {
+        int l = ilog2(n);
+        *n1 = 1 << (l / 2);
+        *n2 = n / *n1;
+    }
 }
 
 /* One forward transform following the reference's dispatch

@@ -118,3 +118,52 @@ def test_sar_pipeline(gpu, precision, rcmc):
     def run():
         rda.process(raw, img)
     replay_matches_eager(torch, run, [img])
+
+
+def test_scratch_growth_after_capture(gpu):
+    """A graph captured at a small size stays valid after eager calls at a
+    larger size have grown the object's scratch (the superseded allocation
+    is kept until the object is destroyed): the convolver adjoint's tap
+    partials and the analyser's spectra and partial sums both grow here."""
+    import torch
+    K, S = 255, 20000
+    taps = np.random.default_rng(2).standard_normal(K).astype(np.float32) / K
+    cv = gpu.FusedConvolver(taps)
+    n, hop, ch = 1024, 256, 2
+    an = gpu.CrossSpectral(n, ch)
+    small, big = 2, 12
+    x = torch.randn((big, S), device="cuda")
+    gy = torch.randn((big, S + K - 1), device="cuda")
+    gx = torch.empty_like(x)
+    gh = torch.empty(K, device="cuda")
+    sig = torch.randn((ch, 40000), device="cuda")
+    pw = torch.empty((ch, n), device="cuda")
+    cr = torch.empty((1, n), dtype=torch.complex64, device="cuda")
+    co = torch.empty((1, n), device="cuda")
+
+    def run_small():
+        cv.backward_device(x, gy, gx, gh, S, small)
+        an.compute_device(sig, 8000, hop, None, pw, cr, co)
+    run_small()
+    torch.cuda.synchronize()
+    refs = [t.clone() for t in (gx[:small], gh, pw, cr, co)]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        run_small()
+    # eager calls at larger sizes grow both objects' scratch
+    gx_big = torch.empty_like(x)
+    gh_big = torch.empty_like(gh)
+    cv.backward_device(x, gy, gx_big, gh_big, S, big)
+    an.compute_device(sig, 40000, hop, None, pw, cr, co)
+    torch.cuda.synchronize()
+    for r in range(3):
+        for t in (gx, gh, pw, cr, co):
+            t.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        _ = torch.randn((1 << 22,), device="cuda").sum().item()  # reuse freed memory, if any
+        outs = (gx[:small], gh, pw, cr, co)
+        for o, ref in zip(outs, refs):
+            assert torch.equal(o, ref), f"replay {r} differs"
+    del g

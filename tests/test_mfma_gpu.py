@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
 
 # (precision code, name, rel-L2 bound with complex64 I/O)
 PRECISIONS = [(1, "fp16", 2e-3), (2, "bf16", 2e-2), (3, "fp16_split", 1e-5), (4, "bf16_split", 2e-4)]
+PRECISIONS_BOUND = [(p, b) for p, _, b in PRECISIONS]
 
 
 def make_plan(gpu, prec, io):
@@ -89,3 +90,25 @@ def test_mfma_needs_n4096(gpu):
     p = ctypes.c_void_p()
     assert _lib.load().mifft_plan_create_ex(1024, 3, 0, ctypes.byref(p)) == 4
     assert _lib.load().mifft_plan_create_ex(4096, 0, 1, ctypes.byref(p)) == 4
+
+
+@pytest.mark.parametrize("prec", [1, 2, 3, 4])
+def test_mfma_subnormal_row(gpu, oracle, prec):
+    """A transform whose largest value is subnormal (~1e-39) keeps a finite
+    power-of-two scale: no inf or NaN, and the rows around it are unaffected."""
+    import torch
+    from conftest import uniform_c64
+    plan = make_plan(gpu, prec, 0)
+    x = uniform_c64((4, 4096), 21)
+    x[1] = (x[1] * np.float32(1e-39)).astype(np.complex64)
+    x[1, 5:] = 0
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    run(gpu, plan, xd, yd, 4)
+    y = yd.cpu().numpy()
+    assert np.isfinite(y.view(np.float32)).all()
+    ref = oracle.f64_fft(x.astype(np.complex128))
+    bound = dict(PRECISIONS_BOUND)[prec]
+    assert oracle.rel_l2(y[[0, 2, 3]], ref[[0, 2, 3]]) < bound
+    # the subnormal row: FP16/BF16 keep ~2^-125 * 2^-24 absolute at worst
+    assert np.max(np.abs(y[1] - ref[1])) < 1e-38

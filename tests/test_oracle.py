@@ -20,6 +20,9 @@ import pytest
 from conftest import uniform_c64
 
 SIZES = (64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
+# beyond the reference's table: the four-step procedure of MetalFFT.swift:206-277
+# with n1 = 2^floor(L/2), n2 = 2^ceil(L/2)
+FOURSTEP_SIZES = (32768, 65536, 131072, 262144, 524288, 1 << 20)
 
 
 @pytest.mark.parametrize("n", [64, 128, 256, 512])
@@ -100,7 +103,7 @@ def test_radix8_matches_radix4(oracle):
     assert oracle.rel_l2(b, oracle.f64_fft(x)) < 1e-6
 
 
-@pytest.mark.parametrize("n", [8192, 16384])
+@pytest.mark.parametrize("n", [8192, 16384, *FOURSTEP_SIZES])
 def test_four_step_index_identity(oracle, n):
     # impulse at position p -> X[k] = W_N^{pk}: pins the four-step's
     # transpose order and twiddle exponent (MetalFFT.swift:206-277)
@@ -378,3 +381,38 @@ def test_window_definitions():
     assert np.allclose(k, k[::-1]) and k.max() <= 1.0 + 1e-7
     for x in (0.0, 0.5, 3.0, 8.6, 14.0):
         assert abs(_bessel_i0(x) - scipy.special.i0(x)) < 1e-11 * scipy.special.i0(x)
+
+
+@pytest.mark.parametrize("n", FOURSTEP_SIZES)
+def test_long_four_step_vs_fp64(oracle, n):
+    """Lengths above the reference's table: FP32 four-step vs FP64, DC and
+    impulse known answers, and the round trip."""
+    x = uniform_c64((2, n), n + 1)
+    y = oracle.cpu_fft(x, threads=2)
+    assert oracle.rel_l2(y, oracle.f64_fft(x)) < 1e-6
+    assert oracle.rel_l2(oracle.cpu_fft(y, inverse=True, threads=2), x) < 1e-6
+    d = oracle.cpu_fft(np.ones(n, np.complex64))
+    assert abs(d[0] - n) < 1e-6 * n and np.max(np.abs(d[1:])) < 1e-6 * n
+    i = np.zeros(n, np.complex64)
+    i[0] = 1
+    assert np.max(np.abs(oracle.cpu_fft(i) - 1)) < 1e-6
+
+
+def test_four_step_transposed_split_is_caught(oracle):
+    """A four-step with its two sub-lengths swapped in the final transpose
+    (writing X[k1 + N1 k2] instead of X[k2 + N2 k1]) is a plausible slip: at
+    32768 = 128 x 256 it permutes the bins, and the FP64 comparison above
+    would fail on it."""
+    n, n1, n2 = 32768, 128, 256
+    x = uniform_c64((n,), 5)
+    y = oracle.cpu_fft(x)
+    k = np.arange(n)
+    k2, k1 = k % n2, k // n2
+    wrong = np.empty_like(y)
+    wrong[k1 + n1 * k2] = y[k2 + n2 * k1]
+    assert oracle.rel_l2(wrong, oracle.f64_fft(x)) > 0.5
+
+
+def test_oracle_rejects_beyond_2_20(oracle):
+    with pytest.raises(ValueError):
+        oracle.cpu_fft(np.zeros(1 << 21, np.complex64))
