@@ -428,12 +428,14 @@ def extra_rocfft(m, torch, n=4096, batch=4096):
 def extra_sizes(m, torch):
     """Every supported length at 2^24 complex values per launch (128 MiB in,
     128 MiB out), four rotating sets, the autotuned kernel, graph replay of
-    200 launches: transforms/s and fraction of the HBM peak."""
+    200 launches: transforms/s and fraction of the HBM peak (algorithmic
+    bytes, 16 n per transform; the four-step lengths 32768 ... 2^20 move
+    twice that through HBM in their two passes)."""
     import ctypes
     from metalfft_amd import _lib
     L = _lib.load()
     out = {}
-    for n in m.SUPPORTED_SIZES:
+    for n in m.SUPPORTED_SIZES + m.FOURSTEP_SIZES:
         batch = (1 << 24) // n
         f = m.MetalFFT(n)
         sets = [(torch.randn((batch, n), dtype=torch.complex64, device="cuda"),

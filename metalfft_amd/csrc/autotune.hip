@@ -259,26 +259,51 @@ const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* t
     return q < 0 ? default_variant(n) : vs[q];
 }
 
-const FourStepVariant* choose_fourstep(int n, hipStream_t stream, const float2* tw) {
-    if (mode() == "0") return find_fourstep(n);
-    int c = 0;
-    const FourStepVariant* all = fourstep_variants(&c);
-    std::vector<const FourStepVariant*> vs;
+void choose_long(int n, hipStream_t stream, const float2* tw, const float2* fs_tw,
+                 const StockhamVariant** variant, const FourStepVariant** fourstep) {
+    *variant = nullptr;
+    *fourstep = fs_tw ? find_fourstep(n) : nullptr;
+    if (!*fourstep && tw) *variant = default_variant(n);
+    if (mode() == "0") return;
     std::vector<Candidate> cand;
-    for (int i = 0; i < c; ++i) {
-        if (all[i].n != n) continue;
-        const FourStepVariant* v = &all[i];
-        vs.push_back(v);
-        // out of place: the output buffer holds the intermediate
-        cand.push_back({v->name,
-                        [=](const void* in, void* out, int64_t batch) {
-                            return v->launch(static_cast<const float2*>(in), static_cast<float2*>(out),
-                                             static_cast<float2*>(out), tw, batch, false, stream);
-                        },
-                        true});
+    std::vector<const StockhamVariant*> vs;
+    std::vector<const FourStepVariant*> fs;
+    int c = 0;
+    if (tw) {
+        const StockhamVariant* all = stockham_variants(&c);
+        for (int i = 0; i < c; ++i) {
+            if (all[i].n != n) continue;
+            const StockhamVariant* v = &all[i];
+            cand.push_back({v->name,
+                            [=](const void* in, void* out, int64_t batch) {
+                                return v->launch(static_cast<const float2*>(in), static_cast<float2*>(out),
+                                                 tw, batch, false, stream);
+                            },
+                            true});
+            vs.push_back(v);
+            fs.push_back(nullptr);
+        }
     }
-    const int q = tuned_choice("fourstep fp32 c64 " + std::to_string(n), n, 8, cand, stream, "fourstep");
-    return q < 0 ? find_fourstep(n) : vs[q];
+    if (fs_tw) {
+        const FourStepVariant* all = fourstep_variants(&c);
+        for (int i = 0; i < c; ++i) {
+            if (all[i].n != n) continue;
+            const FourStepVariant* v = &all[i];
+            // out of place: the output buffer holds the intermediate
+            cand.push_back({v->name,
+                            [=](const void* in, void* out, int64_t batch) {
+                                return v->launch(static_cast<const float2*>(in), static_cast<float2*>(out),
+                                                 static_cast<float2*>(out), fs_tw, batch, false, stream);
+                            },
+                            true});
+            vs.push_back(nullptr);
+            fs.push_back(v);
+        }
+    }
+    const int q = tuned_choice("long fp32 c64 " + std::to_string(n), n, 8, cand, stream, "long");
+    if (q < 0) return;
+    *variant = vs[q];
+    *fourstep = fs[q];
 }
 
 }  // namespace mifft
@@ -307,18 +332,21 @@ int mifft_kernel_variants(int64_t n, const char** names, int cap) {
 mifft_status mifft_plan_select_kernel(mifft_plan* plan, const char* name) {
     if (!plan || !name) return mifft::fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
     if (plan->precision == MIFFT_PREC_FP32) {
-        if (plan->fourstep) {
+        if (plan->fs_twiddles) {
             int c = 0;
             const mifft::FourStepVariant* f = mifft::fourstep_variants(&c);
             for (int i = 0; i < c; ++i)
                 if (f[i].n == plan->n && std::strcmp(f[i].name, name) == 0) {
                     plan->fourstep = &f[i];
+                    plan->variant = nullptr;
                     return MIFFT_OK;
                 }
-        } else {
+        }
+        if (plan->twiddles) {
             const mifft::StockhamVariant* v = mifft::by_name(name);
             if (v && v->n == plan->n) {
                 plan->variant = v;
+                plan->fourstep = nullptr;
                 return MIFFT_OK;
             }
         }

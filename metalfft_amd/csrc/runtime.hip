@@ -152,7 +152,7 @@ mifft_status run(const mifft_plan* plan, const float* in, float* out, int64_t ba
             y = static_cast<float2*>(plan->fs_tmp.ptr);
         }
         return check_hip(plan->fourstep->launch(reinterpret_cast<const float2*>(in),
-                                                reinterpret_cast<float2*>(out), y, plan->twiddles,
+                                                reinterpret_cast<float2*>(out), y, plan->fs_twiddles,
                                                 batch, dir == MIFFT_INVERSE, stream),
                          plan->fourstep->name);
     }
@@ -296,11 +296,17 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
         return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
     }
     const bool four = fourstep_size(n);
-    const size_t tw_len = four ? fourstep_table_len((int)n) : twiddle_table_len((int)n);
-    if ((s = check_hip(hipMalloc(&p->twiddles, sizeof(float2) * tw_len), "hipMalloc")) != MIFFT_OK) {
+    const bool single = find_stockham((int)n, 16) || find_stockham((int)n, 32);
+    const size_t st_len = single ? (size_t)twiddle_table_len((int)n) : 0;  // even: 16-byte rows
+    const size_t fs_len = four ? (size_t)fourstep_table_len((int)n) : 0;
+    float2* tw_all = nullptr;
+    if ((s = check_hip(hipMalloc(&tw_all, sizeof(float2) * (st_len + fs_len)), "hipMalloc")) != MIFFT_OK) {
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
     }
+    p->twiddles = single ? tw_all : nullptr;
+    p->fs_twiddles = four ? tw_all + st_len : nullptr;
+    p->tw_alloc = tw_all;
     // real-input tables for the single-pass lengths only (the real entry
     // points report kernelNotFound for four-step plans)
     if (!four && (s = check_hip(hipMalloc(&p->real_tw, sizeof(float2) * real_table_len((int)n)),
@@ -308,9 +314,10 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_ALLOCATION, "Buffer allocation failed");
     }
-    if ((s = check_hip(four ? launch_fourstep_twiddles(p->twiddles, (int)n, p->stream)
-                            : launch_twiddles(p->twiddles, (int)n, p->stream),
-                       "twiddles")) != MIFFT_OK ||
+    if ((single && (s = check_hip(launch_twiddles(p->twiddles, (int)n, p->stream), "twiddles")) !=
+                       MIFFT_OK) ||
+        (four && (s = check_hip(launch_fourstep_twiddles(p->fs_twiddles, (int)n, p->stream),
+                                "four-step twiddles")) != MIFFT_OK) ||
         (!four && (s = check_hip(launch_real_table(p->real_tw, (int)n, p->stream), "real-input table")) !=
                       MIFFT_OK) ||
         (s = check_hip(hipStreamSynchronize(p->stream), "hipStreamSynchronize")) != MIFFT_OK) {
@@ -318,8 +325,8 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
         return s;
     }
     if (four) {
-        p->fourstep = choose_fourstep((int)n, p->stream, p->twiddles);
-        if (!p->fourstep) {
+        choose_long((int)n, p->stream, p->twiddles, p->fs_twiddles, &p->variant, &p->fourstep);
+        if (!p->fourstep && !p->variant) {
             mifft_plan_destroy(p);
             return fail(MIFFT_ERR_KERNEL_NOT_FOUND, "HIP kernel not found for size " + std::to_string(n));
         }
@@ -344,7 +351,7 @@ mifft_status mifft_plan_destroy(mifft_plan* plan) {
     plan->in.release();
     plan->out.release();
     plan->fs_tmp.release();
-    if (plan->twiddles) (void)hipFree(plan->twiddles);
+    if (plan->tw_alloc) (void)hipFree(plan->tw_alloc);
     if (plan->real_tw) (void)hipFree(plan->real_tw);
     if (plan->stream) (void)hipStreamDestroy(plan->stream);
     delete plan;

@@ -48,7 +48,11 @@ struct DeviceBuffer {
 // Kernel variant per size: picked by the autotuner (timed on first use and
 // cached on disk) or the built-in default.
 const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* tw);
-const FourStepVariant* choose_fourstep(int n, hipStream_t stream, const float2* tw);
+// Lengths above 16384: the fastest of the single-workgroup variants of n
+// (tw: their table, or null) and the four-step variants (fs_tw, or null);
+// exactly one of *variant / *fourstep is set.
+void choose_long(int n, hipStream_t stream, const float2* tw, const float2* fs_tw,
+                 const StockhamVariant** variant, const FourStepVariant** fourstep);
 
 }  // namespace mifft
 
@@ -58,8 +62,11 @@ struct mifft_plan {
     int io = 0;         // mifft_io
     const mifft::StockhamVariant* variant = nullptr;   // n <= 16384
     const mifft::FourStepVariant* fourstep = nullptr;  // n > 16384
-    float2* twiddles = nullptr;  // W_N^k, k < N, then pass rows (twiddle_table_len);
-                                 // four-step: fourstep_table_len
+    float2* twiddles = nullptr;     // single-workgroup kernels: W_N^k, k < N, then the
+                                    // pass rows (twiddle_table_len); null if none exists
+    float2* fs_twiddles = nullptr;  // four-step kernels (fourstep_table_len), in the same
+                                    // allocation after `twiddles`; null below 32768
+    float2* tw_alloc = nullptr;     // the allocation holding both
     float2* real_tw = nullptr;   // real-input transforms (real_table_len)
     hipStream_t stream = nullptr;
     mifft::DeviceBuffer in, out;

@@ -174,6 +174,10 @@ static const StockhamVariant kVariants[] = {
     MIFFT_VP3(16384),
     MIFFT_VH(2048, 4),  MIFFT_VH(4096, 4), MIFFT_VH(8192, 4),
     MIFFT_VH(16384, 4),
+    // 32768 in one workgroup: 1024 threads x 32 values, 128 KiB of LDS
+    // exchanged in halves (one transform per CU); the four-step kernels are
+    // the other candidates for this length
+    MIFFT_VH(32768, 4),
 };
 #undef MIFFT_V
 #undef MIFFT_VS
@@ -233,6 +237,7 @@ int twiddle_table_len(int n) {
         case 4096: return 4096 + pass_rows<4096>();
         case 8192: return 8192 + pass_rows<8192>();
         case 16384: return 16384 + pass_rows<16384>();
+        case 32768: return 32768 + pass_rows<32768>();
         default: return n;
     }
 }
@@ -252,6 +257,7 @@ hipError_t launch_twiddles(float2* tw, int n, hipStream_t stream) {
         case 4096: return launch_pass_rows<4096>(tw + n, stream);
         case 8192: return launch_pass_rows<8192>(tw + n, stream);
         case 16384: return launch_pass_rows<16384>(tw + n, stream);
+        case 32768: return launch_pass_rows<32768>(tw + n, stream);
         default: return hipSuccess;
     }
 }

@@ -1,5 +1,6 @@
 """Four-step transforms above 16384 points (32768 ... 2^20) on the GPU,
-through the C-ABI: every compiled variant against FP64 and the oracle's
+through the C-ABI (at 32768 also the single-workgroup half-LDS kernel, the
+autotuner's other candidate): every compiled variant against FP64 and the oracle's
 FP32 four-step restatement of MetalFFT.swift:206-277, within 1e-6 rel-L2;
 inverse round trip; in-place == out-of-place and batch == single bit for
 bit; the host entry points; hipGraph replay."""
@@ -25,9 +26,9 @@ def test_every_fourstep_variant(gpu, oracle, n):
     ref = oracle.f64_fft(x)
     cpu = oracle.cpu_fft(x, threads=8)
     fft = gpu.MetalFFT(n)
-    assert fft.kernel_name.startswith("fourstep_")
     names = gpu.kernel_variants(n)
-    assert names
+    assert any(s.startswith("fourstep_") for s in names)
+    assert fft.kernel_name in names
     src = torch.from_numpy(x).cuda()
     for name in names:
         fft.select_kernel(name)

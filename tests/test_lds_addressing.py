@@ -78,7 +78,7 @@ def test_butterfly_write_identity(n):
                         assert swz(sb + j * s) == swz(sb) ^ swz(j * s)
 
 
-@pytest.mark.parametrize("n", [2048, 4096, 8192, 16384])
+@pytest.mark.parametrize("n", [2048, 4096, 8192, 16384, 32768])
 def test_split_exchange_halves(n):
     """E = 32: butterfly i of every thread writes half i // (B/2), each half
     exactly once, and half h is read back as v[m], m in [16h, 16h + 16)."""
