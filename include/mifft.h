@@ -267,6 +267,10 @@ typedef struct mifft_ragged_plan mifft_ragged_plan;
 #define MIFFT_RAGGED_PER_SIZE 1   /* one launch per distinct length (baseline) */
 #define MIFFT_RAGGED_GRAPH 2      /* capture the launch(es) in a hipGraph once per
                                      (in, out, direction, mode) and replay it */
+#define MIFFT_RAGGED_QUEUE 4      /* one persistent launch for every length: workgroups
+                                     take tickets (a few transforms of one length, the
+                                     longest lengths first) from a device counter; the
+                                     plan's counter makes calls on one plan exclusive */
 
 /* lengths[i] in {64, ..., 16384} (else MIFFT_ERR_UNSUPPORTED_SIZE with
  * batch_index = i); offsets[i] = first complex element of transform i, or

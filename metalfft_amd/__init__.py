@@ -6,7 +6,7 @@ API.  Importing does not touch the GPU; the first plan creation does.
 """
 from ._lib import FFTError, FORWARD, INVERSE  # noqa: F401
 from .api import (MetalFFT, MultiChannelFFT, PrecisionFFT, RaggedFFT,  # noqa: F401
-                  FOURSTEP_SIZES, RAGGED_GRAPH, RAGGED_PER_SIZE, RAGGED_PERSISTENT,
+                  FOURSTEP_SIZES, RAGGED_GRAPH, RAGGED_PER_SIZE, RAGGED_PERSISTENT, RAGGED_QUEUE,
                   SUPPORTED_SIZES,
                   device_name, kernel_variants)
 from .dsp import (PSD, STFT, AnalysisResult, Convolver, Correlator,  # noqa: F401
@@ -17,6 +17,6 @@ from .sar import RangeDoppler, RdaParams  # noqa: F401
 __all__ = ["FFTError", "FORWARD", "INVERSE", "MetalFFT", "MultiChannelFFT",
            "SUPPORTED_SIZES", "FOURSTEP_SIZES", "device_name", "kernel_variants", "Convolver", "FusedConvolver",
            "Correlator", "Precision", "RaggedFFT", "PrecisionFFT", "RAGGED_GRAPH",
-           "RAGGED_PER_SIZE", "RAGGED_PERSISTENT", "RangeDoppler", "RdaParams", "CrossSpectral",
+           "RAGGED_PER_SIZE", "RAGGED_PERSISTENT", "RAGGED_QUEUE", "RangeDoppler", "RdaParams", "CrossSpectral",
            "CrossSpectralResult", "cross_spectral", "window_coefficients", "STFT", "STFTFrame",
            "FFTAnalyzer", "AnalysisResult", "PSD", "to_decibels"]

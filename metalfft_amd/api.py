@@ -265,6 +265,7 @@ class MultiChannelFFT:
 RAGGED_PERSISTENT = 0
 RAGGED_PER_SIZE = 1
 RAGGED_GRAPH = 2
+RAGGED_QUEUE = 4
 
 
 class RaggedFFT:

@@ -212,6 +212,118 @@ hipError_t launch_class(const float2* in, float2* out, const float2* tw_all, con
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------- work queue
+// One persistent launch of 512-thread workgroups for every length.  A
+// ticket is `count` transforms of one length 2^L, taken in units of what one
+// 512-thread pass holds (queue_unit: 16384: one, half-LDS form with every
+// thread; 8192: two, half form on 256 threads each; 4096 and shorter: 8192
+// elements, 16 values per thread), all within 64 KiB of LDS.  Tickets are
+// sorted by length, largest first; each workgroup takes the next ticket
+// from one device counter (an atomic increment by one lane, broadcast
+// through LDS), drawing its following ticket while it works on the current
+// one.  A workgroup that draws past the end leaves; the last one to leave
+// (a second counter) sets both counters back to zero, so the queue is ready
+// for the next launch or graph replay without a memset.
+constexpr int queue_unit(int l) { return l == 14 ? 1 : l == 13 ? 2 : 8192 >> l; }
+
+template <int L, bool INV>
+__device__ __forceinline__ void queue_unit_body(const float2* in, float2* out, const float2* tw,
+                                                const int64_t* offs, int64_t first, int count, float2* lds) {
+    constexpr int N = 1 << L;
+    const int tid = fresh((int)threadIdx.x);  // index arithmetic redone per unit, not hoisted
+    if constexpr (L == 14) {
+        const int64_t off = offs[first];
+        half_transform<N, INV, true>(in + off, out + off, tw, lds, tid, true);
+    } else if constexpr (L == 13) {
+        const int h = tid >> 8, t = tid & 255;
+        const int64_t off = offs[first + (h < count ? h : count - 1)];
+        half_transform<N, INV, true>(in + off, out + off, tw, lds + h * (N / 2), t, h < count);
+    } else {
+        constexpr int T = N / 16;
+        const int local = tid / T, t = tid % T;
+        // a slot past `count` transforms the unit's last transform again and
+        // stores nothing
+        const int64_t off = offs[first + (local < count ? local : count - 1)];
+        float2 v[16];
+        load_column<N, 16, INV, true>(v, in + off, t, true);
+        PreBases<N, 16> pb;
+        pb.load_packed(tw, t);
+        stockham_regs<N, 16, LdsSwz, 0, true, true, false, 2>(v, lds + local * N, t, tw, pb.w);
+        if (local < count) store_column<N, 16, INV, true>(v, out + off, t);
+    }
+}
+
+template <int L, bool INV>
+__device__ __forceinline__ void queue_ticket(const float2* in, float2* out, const float2* tw_all,
+                                             const int64_t* offs, int64_t first, int count, float2* lds) {
+    constexpr int U = queue_unit(L);
+    constexpr int kOff = table_off(L);
+    for (int b = 0; b < count; b += U) {  // count is uniform
+        if (b > 0) lds_barrier();        // the previous unit's LDS reads are done
+        queue_unit_body<L, INV>(in, out, tw_all + kOff, offs, first + b, count - b < U ? count - b : U, lds);
+    }
+}
+
+template <bool INV>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_ragged_queue(const float2* in, float2* out, const float2* __restrict__ tw_all,
+               const RaggedTile* __restrict__ tickets, const int64_t* __restrict__ offs, int ntickets,
+               int* queue) {
+    __shared__ float2 lds[8192];
+    __shared__ int s_next;
+    if (threadIdx.x == 0) s_next = atomicAdd(&queue[0], 1);
+    __syncthreads();
+    int cur = __builtin_amdgcn_readfirstlane(s_next);
+    while (cur < ntickets) {  // uniform
+        int nxt = 0;
+        if (threadIdx.x == 0) nxt = atomicAdd(&queue[0], 1);  // in flight during this ticket
+        const RaggedTile d = tickets[cur];
+        const int64_t first = d.first;
+        const int count = d.count;
+        switch (d.log2n) {
+            case 14: queue_ticket<14, INV>(in, out, tw_all, offs, first, count, lds); break;
+            case 13: queue_ticket<13, INV>(in, out, tw_all, offs, first, count, lds); break;
+            case 12: queue_ticket<12, INV>(in, out, tw_all, offs, first, count, lds); break;
+            case 11: queue_ticket<11, INV>(in, out, tw_all, offs, first, count, lds); break;
+            case 10: queue_ticket<10, INV>(in, out, tw_all, offs, first, count, lds); break;
+            case 9: queue_ticket<9, INV>(in, out, tw_all, offs, first, count, lds); break;
+            case 8: queue_ticket<8, INV>(in, out, tw_all, offs, first, count, lds); break;
+            case 7: queue_ticket<7, INV>(in, out, tw_all, offs, first, count, lds); break;
+            default: queue_ticket<6, INV>(in, out, tw_all, offs, first, count, lds); break;
+        }
+        __syncthreads();  // every read of s_next and of this ticket's LDS is done
+        if (threadIdx.x == 0) s_next = nxt;
+        __syncthreads();
+        cur = __builtin_amdgcn_readfirstlane(s_next);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();  // this workgroup's draws precede its exit count
+        if (atomicAdd(&queue[1], 1) == (int)gridDim.x - 1) {
+            atomicExch(&queue[0], 0);
+            atomicExch(&queue[1], 0);
+        }
+    }
+}
+
+hipError_t launch_queue(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tickets,
+                        const int64_t* offs, int ntickets, int* queue, bool inverse, hipStream_t st) {
+    if (ntickets <= 0) return hipSuccess;
+    static const int per_cu = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_ragged_queue<false>, 512, 0) != hipSuccess || b < 1)
+            b = 1;
+        return b;
+    }();
+    const int grid = std::min(ntickets, ragged_cus() * per_cu);
+    if (inverse)
+        hipLaunchKernelGGL(k_ragged_queue<true>, dim3(grid), dim3(512), 0, st, in, out, tw_all, tickets, offs,
+                           ntickets, queue);
+    else
+        hipLaunchKernelGGL(k_ragged_queue<false>, dim3(grid), dim3(512), 0, st, in, out, tw_all, tickets, offs,
+                           ntickets, queue);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 int ragged_twiddles_len() { return kTablesLen; }
@@ -255,6 +367,10 @@ mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool 
     const RaggedTile* tiles = static_cast<const RaggedTile*>(rp->d_tiles.ptr);
     const int64_t* offs = static_cast<const int64_t*>(rp->d_offsets.ptr);
     const float2* tw = static_cast<const float2*>(rp->d_twiddles.ptr);
+    if (mode & MIFFT_RAGGED_QUEUE)
+        return check_hip(launch_queue(in, out, tw, static_cast<const RaggedTile*>(rp->d_tickets.ptr), offs,
+                                      rp->ntickets, static_cast<int*>(rp->d_queue.ptr), inverse, st),
+                         "ragged fft (work queue)");
     // lengths >= 4096: one launch each (their tiles hold one transform, so
     // the tile range is the sorted transform range)
     int small_begin = rp->ntiles;
@@ -276,6 +392,31 @@ mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool 
                                              st),
                          "ragged fft");
     return MIFFT_OK;
+}
+
+// Work-queue tickets: the transforms of each length (sorted order, largest
+// length first) in runs of units_per_ticket[l - 6] * queue_unit(l).
+std::vector<RaggedTile> build_tickets(const int64_t* lengths, const std::vector<int64_t>& order, int64_t count,
+                                      const int* units_per_ticket) {
+    std::vector<RaggedTile> t;
+    int64_t i = 0;
+    while (i < count) {
+        const int64_t n = lengths[order[i]];
+        int l = 0;
+        while ((int64_t(1) << l) < n) ++l;
+        int64_t j = i;
+        while (j < count && lengths[order[j]] == n) ++j;
+        const int64_t per = (int64_t)queue_unit(l) * units_per_ticket[l - 6];
+        for (int64_t a = i; a < j; a += per) {
+            RaggedTile d;
+            d.log2n = l;
+            d.count = (int)std::min<int64_t>(per, j - a);
+            d.first = a;
+            t.push_back(d);
+        }
+        i = j;
+    }
+    return t;
 }
 
 void drop_graph(mifft_ragged_plan* rp) {
@@ -343,6 +484,8 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
     begins.push_back((int)tiles.size());
 
     mifft_ragged_plan* rp = new mifft_ragged_plan();
+    std::vector<RaggedTile> tickets = build_tickets(lengths, order, count, rp->units_per_ticket);
+    rp->ntickets = (int)tickets.size();
     rp->count = count;
     rp->extent = extent;
     rp->ntiles = (int)tiles.size();
@@ -359,7 +502,14 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
         mifft_ragged_plan_destroy(rp);
         return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
     }
-    if ((s = rp->d_tiles.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tiles.size()))) != MIFFT_OK ||
+    if ((s = rp->d_tickets.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tickets.size()))) != MIFFT_OK ||
+        (s = rp->d_queue.reserve(2 * sizeof(int))) != MIFFT_OK ||
+        (s = check_hip(hipMemset(rp->d_queue.ptr, 0, 2 * sizeof(int)), "hipMemset")) != MIFFT_OK ||
+        (!tickets.empty() &&
+         (s = check_hip(hipMemcpy(rp->d_tickets.ptr, tickets.data(), sizeof(RaggedTile) * tickets.size(),
+                                  hipMemcpyHostToDevice),
+                        "hipMemcpy")) != MIFFT_OK) ||
+        (s = rp->d_tiles.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tiles.size()))) != MIFFT_OK ||
         (s = rp->d_offsets.reserve(sizeof(int64_t) * std::max<int64_t>(1, count))) != MIFFT_OK ||
         (s = rp->d_twiddles.reserve(sizeof(float2) * ragged_twiddles_len())) != MIFFT_OK) {
         mifft_ragged_plan_destroy(rp);
@@ -387,6 +537,8 @@ mifft_status mifft_ragged_plan_destroy(mifft_ragged_plan* rp) {
     if (rp->stream) (void)hipStreamSynchronize(rp->stream);
     drop_graph(rp);
     rp->d_tiles.release();
+    rp->d_tickets.release();
+    rp->d_queue.release();
     rp->d_offsets.release();
     rp->d_twiddles.release();
     if (rp->stream) (void)hipStreamDestroy(rp->stream);
@@ -399,6 +551,7 @@ int64_t mifft_ragged_elements(const mifft_ragged_plan* rp) { return rp ? rp->ele
 int64_t mifft_ragged_launches(const mifft_ragged_plan* rp, int mode) {
     if (!rp) return -1;
     if (rp->ntiles == 0) return 0;
+    if (mode & MIFFT_RAGGED_QUEUE) return 1;
     if (mode & MIFFT_RAGGED_PER_SIZE) return (int64_t)rp->size_tile_begin.size() - 1;
     int64_t n = 0;
     bool small = false;
@@ -414,8 +567,10 @@ mifft_status mifft_ragged_execute_device(mifft_ragged_plan* rp, const float* in,
     if (!rp) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null plan");
     if (dir != MIFFT_FORWARD && dir != MIFFT_INVERSE)
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "direction must be MIFFT_FORWARD or MIFFT_INVERSE");
-    if (mode & ~(MIFFT_RAGGED_PER_SIZE | MIFFT_RAGGED_GRAPH))
+    if (mode & ~(MIFFT_RAGGED_PER_SIZE | MIFFT_RAGGED_GRAPH | MIFFT_RAGGED_QUEUE))
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown mode bits");
+    if ((mode & MIFFT_RAGGED_PER_SIZE) && (mode & MIFFT_RAGGED_QUEUE))
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "MIFFT_RAGGED_PER_SIZE and MIFFT_RAGGED_QUEUE exclude each other");
     if (rp->count == 0) return MIFFT_OK;
     if (!in || !out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null buffer");
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -30,6 +30,11 @@ struct mifft_ragged_plan {
     mifft::RaggedRanges ranges = {};   // the same ranges indexed by log2 length
     std::vector<int64_t> size_first;   // sorted index of each length's first transform
     mifft::DeviceBuffer d_tiles, d_offsets, d_twiddles;
+    // work-queue form: tickets (length, count, first) sorted largest length
+    // first, and the two device counters (next ticket, workgroups done)
+    int ntickets = 0;
+    int units_per_ticket[9] = {1, 1, 1, 1, 1, 1, 1, 1, 1};  // by log2 length - 6
+    mifft::DeviceBuffer d_tickets, d_queue;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

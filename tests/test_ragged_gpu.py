@@ -1,7 +1,8 @@
-"""Ragged batches (lengths 64 ... 16384 mixed): one launch per length of
-4096 and longer plus one persistent launch for the shorter ones, or one
-launch per size, optionally replayed from a hipGraph, against FP64 per
-transform."""
+"""Ragged batches (lengths 64 ... 16384 mixed): one persistent launch fed by
+a device work queue (tickets of one length, longest first), one launch per
+length of 4096 and longer plus one persistent launch for the shorter ones,
+or one launch per size, optionally replayed from a hipGraph, against FP64
+per transform; every form gives the same bits."""
 import numpy as np
 import pytest
 
@@ -127,6 +128,62 @@ def test_graph_replays_match_eager(gpu, oracle):
         _ = (xd * 2).sum().item()  # unrelated eager work between replays
 
 
+@pytest.mark.parametrize("lengths", [[64, 4096, 128, 1024] * 40, [8192, 64] * 31, [16384] * 5,
+                                     [16384, 8192] * 7 + [8192], [2048] * 3 + [8192], [64] * 129,
+                                     [4096] * 3, [256, 512, 1024, 2048] * 50])
+def test_queue_matches_other_forms(gpu, oracle, lengths):
+    """The work-queue launch equals the per-length launches bit for bit
+    (same transform code per length), including partial tickets (an odd
+    number of 8192 or 4096 transforms, a short last tile), and is one launch."""
+    import torch
+    lengths = np.array(lengths, np.int64)
+    x = ragged_data(lengths, 6)
+    rp = gpu.RaggedFFT(lengths)
+    assert rp.launches(gpu.RAGGED_QUEUE) == 1
+    xd = torch.from_numpy(x).cuda()
+    ref = torch.empty_like(xd)
+    rp.execute(xd, ref, mode=gpu.RAGGED_PERSISTENT)
+    q = torch.empty_like(xd)
+    for rep in range(3):  # the queue counters reset themselves between launches
+        q.fill_(float("nan"))
+        rp.execute(xd, q, mode=gpu.RAGGED_QUEUE)
+        torch.cuda.synchronize()
+        assert torch.equal(q, ref), rep
+    check_all(oracle, lengths, x, q.cpu().numpy())
+    back = torch.empty_like(xd)
+    rp.execute(q, back, direction=gpu.INVERSE, mode=gpu.RAGGED_QUEUE)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(back.cpu().numpy(), x) < 1e-6
+
+
+def test_queue_graph_and_in_place(gpu, oracle):
+    import torch
+    lengths = np.array([4096, 64, 16384, 64, 8192], np.int64)
+    offsets = np.array([100, 0, 5000, 30000, 40000], np.int64)
+    rp = gpu.RaggedFFT(lengths, offsets)
+    buf = (np.random.default_rng(8).standard_normal(rp.extent) * (1 - 1j)).astype(np.complex64)
+    d = torch.from_numpy(buf.copy()).cuda()
+    rp.execute(d, d, mode=gpu.RAGGED_QUEUE)  # in place
+    torch.cuda.synchronize()
+    out = d.cpu().numpy()
+    for n, o in zip(lengths, offsets):
+        assert oracle.rel_l2(out[o:o + n], oracle.f64_fft(buf[o:o + n])) < 1e-6
+    assert np.array_equal(out[64:100], buf[64:100])
+    src = torch.from_numpy(buf).cuda()
+    eager = torch.empty_like(src)
+    rp.execute(src, eager, mode=gpu.RAGGED_QUEUE)
+    g = torch.empty_like(src)
+    for r in range(4):
+        g.fill_(float("nan"))
+        rp.execute(src, g, mode=gpu.RAGGED_QUEUE | gpu.RAGGED_GRAPH)
+        torch.cuda.synchronize()
+        for n, o in zip(lengths, offsets):
+            assert torch.equal(g[o:o + n], eager[o:o + n]), r
+        _ = (src * 2).sum().item()
+    with pytest.raises(gpu.FFTError):
+        rp.execute(src, g, mode=gpu.RAGGED_QUEUE | gpu.RAGGED_PER_SIZE)
+
+
 def test_config2_ten_thousand(gpu, oracle):
     """10,000 transforms, lengths log-uniform over {64..16384} (seed 1), N(0,1)
     values: forward + inverse, persistent vs per-size; FP64 on a sample."""
@@ -139,10 +196,13 @@ def test_config2_ten_thousand(gpu, oracle):
     xd = torch.from_numpy(x).cuda()
     y1 = torch.empty_like(xd)
     y2 = torch.empty_like(xd)
+    y3 = torch.empty_like(xd)
     rp.execute(xd, y1, mode=gpu.RAGGED_PERSISTENT)
     rp.execute(xd, y2, mode=gpu.RAGGED_PER_SIZE)
+    rp.execute(xd, y3, mode=gpu.RAGGED_QUEUE)
     torch.cuda.synchronize()
     assert torch.equal(y1, y2)  # same math, different launch structure
+    assert torch.equal(y1, y3)
     check_all(oracle, lengths, x, y1.cpu().numpy(), stride=97)
     z = torch.empty_like(xd)
     rp.execute(y1, z, direction=gpu.INVERSE)
