@@ -273,8 +273,11 @@ def extra_ragged(m, torch, count=10000, cpu=True):
     y = torch.empty_like(x)
     z = torch.empty_like(x)
     out = {"transforms": count, "elements": tot, "bytes_per_direction": 16 * tot}
-    for name, mode in (("persistent", m.RAGGED_PERSISTENT), ("per_size", m.RAGGED_PER_SIZE),
-                       ("persistent_graph", m.RAGGED_PERSISTENT | m.RAGGED_GRAPH),
+    for name, mode in (("one_launch", m.RAGGED_DEFAULT),
+                       ("one_launch_graph", m.RAGGED_DEFAULT | m.RAGGED_GRAPH),
+                       ("work_queue", m.RAGGED_QUEUE), ("work_queue_graph", m.RAGGED_QUEUE | m.RAGGED_GRAPH),
+                       ("split", m.RAGGED_SPLIT), ("split_graph", m.RAGGED_SPLIT | m.RAGGED_GRAPH),
+                       ("per_size", m.RAGGED_PER_SIZE),
                        ("per_size_graph", m.RAGGED_PER_SIZE | m.RAGGED_GRAPH)):
         def step():
             rp.execute(x, y, m.FORWARD, mode)

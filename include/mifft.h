@@ -261,16 +261,22 @@ MIFFT_API mifft_status mifft_correlator_cross_device(const mifft_correlator *c, 
  * MetalFFT.forward(batch:) rejects uneven lengths, MetalFFT.swift:108-114). */
 typedef struct mifft_ragged_plan mifft_ragged_plan;
 
-/* Execution mode bits for mifft_ragged_execute_device. */
-#define MIFFT_RAGGED_PERSISTENT 0 /* one launch per length of 4096 and longer, one
-                                     persistent launch for all shorter lengths */
+/* Execution mode bits for mifft_ragged_execute_device.  Tickets: the
+ * transforms of one length that one 512-thread workgroup pass holds (one
+ * 16384, two 8192, or 8192 / 4096 elements of a shorter length in the tile
+ * shape measured at plan creation), sorted longest length first. */
+#define MIFFT_RAGGED_DEFAULT 0    /* one launch, one workgroup per ticket: the hardware
+                                     dispatcher starts them in ticket order as CU slots
+                                     free up */
 #define MIFFT_RAGGED_PER_SIZE 1   /* one launch per distinct length (baseline) */
 #define MIFFT_RAGGED_GRAPH 2      /* capture the launch(es) in a hipGraph once per
                                      (in, out, direction, mode) and replay it */
-#define MIFFT_RAGGED_QUEUE 4      /* one persistent launch for every length: workgroups
-                                     take tickets (a few transforms of one length, the
-                                     longest lengths first) from a device counter; the
-                                     plan's counter makes calls on one plan exclusive */
+#define MIFFT_RAGGED_QUEUE 4      /* one persistent launch: the resident workgroups take
+                                     tickets from a device counter (the plan's counter
+                                     makes calls on one plan exclusive) */
+#define MIFFT_RAGGED_SPLIT 8      /* one launch per length of 4096 and longer, one
+                                     persistent launch of 4096-element tiles for the
+                                     shorter lengths */
 
 /* lengths[i] in {64, ..., 16384} (else MIFFT_ERR_UNSUPPORTED_SIZE with
  * batch_index = i); offsets[i] = first complex element of transform i, or

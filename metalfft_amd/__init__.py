@@ -6,7 +6,8 @@ API.  Importing does not touch the GPU; the first plan creation does.
 """
 from ._lib import FFTError, FORWARD, INVERSE  # noqa: F401
 from .api import (MetalFFT, MultiChannelFFT, PrecisionFFT, RaggedFFT,  # noqa: F401
-                  FOURSTEP_SIZES, RAGGED_GRAPH, RAGGED_PER_SIZE, RAGGED_PERSISTENT, RAGGED_QUEUE,
+                  FOURSTEP_SIZES, RAGGED_DEFAULT, RAGGED_GRAPH, RAGGED_PER_SIZE, RAGGED_QUEUE,
+                  RAGGED_SPLIT,
                   SUPPORTED_SIZES,
                   device_name, kernel_variants)
 from .dsp import (PSD, STFT, AnalysisResult, Convolver, Correlator,  # noqa: F401
@@ -17,6 +18,6 @@ from .sar import RangeDoppler, RdaParams  # noqa: F401
 __all__ = ["FFTError", "FORWARD", "INVERSE", "MetalFFT", "MultiChannelFFT",
            "SUPPORTED_SIZES", "FOURSTEP_SIZES", "device_name", "kernel_variants", "Convolver", "FusedConvolver",
            "Correlator", "Precision", "RaggedFFT", "PrecisionFFT", "RAGGED_GRAPH",
-           "RAGGED_PER_SIZE", "RAGGED_PERSISTENT", "RAGGED_QUEUE", "RangeDoppler", "RdaParams", "CrossSpectral",
+           "RAGGED_PER_SIZE", "RAGGED_DEFAULT", "RAGGED_QUEUE", "RAGGED_SPLIT", "RangeDoppler", "RdaParams", "CrossSpectral",
            "CrossSpectralResult", "cross_spectral", "window_coefficients", "STFT", "STFTFrame",
            "FFTAnalyzer", "AnalysisResult", "PSD", "to_decibels"]

@@ -262,20 +262,21 @@ class MultiChannelFFT:
         return self._fft.inverse_batch(inputs)
 
 
-RAGGED_PERSISTENT = 0
-RAGGED_PER_SIZE = 1
-RAGGED_GRAPH = 2
-RAGGED_QUEUE = 4
+RAGGED_DEFAULT = 0   # one launch, one workgroup per ticket (longest lengths first)
+RAGGED_PER_SIZE = 1  # one launch per distinct length
+RAGGED_GRAPH = 2     # hipGraph capture and replay (combines with the others)
+RAGGED_QUEUE = 4     # one persistent launch fed by a device ticket counter
+RAGGED_SPLIT = 8     # one launch per length >= 4096 + one persistent launch of short tiles
 
 
 class RaggedFFT:
-    """Transforms of different lengths (64 ... 16384) in a few launches.
+    """Transforms of different lengths (64 ... 16384) in one launch.
 
     ``lengths[i]`` complex values of transform i start at ``offsets[i]``
     (default: packed back to back).  ``execute`` runs all of them forward or
-    inverse on device buffers; ``mode`` picks one launch per length of 4096
-    and longer plus one persistent launch for the shorter lengths (default),
-    one launch per length, and/or hipGraph replay."""
+    inverse on device buffers; ``mode`` picks the launch structure (default:
+    one launch of length-sorted tickets; RAGGED_PER_SIZE, RAGGED_QUEUE,
+    RAGGED_SPLIT) and/or hipGraph replay (RAGGED_GRAPH)."""
 
     def __init__(self, lengths: Sequence[int], offsets: Sequence[int] | None = None):
         n = len(lengths)
@@ -290,10 +291,10 @@ class RaggedFFT:
         self.extent = int(L.mifft_ragged_extent(self._p))
         self.elements = int(L.mifft_ragged_elements(self._p))
 
-    def launches(self, mode: int = RAGGED_PERSISTENT) -> int:
+    def launches(self, mode: int = RAGGED_DEFAULT) -> int:
         return int(_lib.load().mifft_ragged_launches(self._p, int(mode)))
 
-    def execute(self, src, dst, direction: int = FORWARD, mode: int = RAGGED_PERSISTENT,
+    def execute(self, src, dst, direction: int = FORWARD, mode: int = RAGGED_DEFAULT,
                 stream=None) -> None:
         need = 8 * self.extent
         _lib.check(_lib.load().mifft_ragged_execute_device(

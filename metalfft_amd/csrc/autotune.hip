@@ -137,13 +137,6 @@ bool tune_log() {
     return lg && *lg && *lg != '0';
 }
 
-// One timing candidate: a launch of `batch` transforms in -> out.
-struct Candidate {
-    const char* name;
-    std::function<hipError_t(const void* in, void* out, int64_t batch)> run;
-    bool prefer;  // stateless one-shot form, kept when within 3 % of the fastest
-};
-
 // Times the candidates of length n (elements of elem_bytes in HBM) and
 // returns the index of the one to use (-1 if none ran).
 // Timed as launches of 2^24 / n transforms (128 MiB in, 128 MiB out for
@@ -216,6 +209,8 @@ int time_candidates(int n, size_t elem_bytes, const std::vector<Candidate>& cand
     return best;
 }
 
+}  // namespace
+
 // The table entry for `key`, or the timed choice among `cand` (recorded).
 // Returns the candidate index, or -1 (caller falls back to its default).
 int tuned_choice(const std::string& key, int n, size_t elem_bytes, const std::vector<Candidate>& cand,
@@ -236,7 +231,8 @@ int tuned_choice(const std::string& key, int n, size_t elem_bytes, const std::ve
     save_locked(key, cand[q].name, ns);
     return q;
 }
-}  // namespace
+
+bool tuning_enabled() { return mode() != "0"; }
 
 const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* tw) {
     if (mode() == "0") return default_variant(n);

@@ -212,89 +212,155 @@ hipError_t launch_class(const float2* in, float2* out, const float2* tw_all, con
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------- work queue
-// One persistent launch of 512-thread workgroups for every length.  A
-// ticket is `count` transforms of one length 2^L, taken in units of what one
-// 512-thread pass holds (queue_unit: 16384: one, half-LDS form with every
-// thread; 8192: two, half form on 256 threads each; 4096 and shorter: 8192
-// elements, 16 values per thread), all within 64 KiB of LDS.  Tickets are
-// sorted by length, largest first; each workgroup takes the next ticket
-// from one device counter (an atomic increment by one lane, broadcast
-// through LDS), drawing its following ticket while it works on the current
-// one.  A workgroup that draws past the end leaves; the last one to leave
-// (a second counter) sets both counters back to zero, so the queue is ready
-// for the next launch or graph replay without a memset.
-constexpr int queue_unit(int l) { return l == 14 ? 1 : l == 13 ? 2 : 8192 >> l; }
+// ---------------------------------------------------------------- tickets
+// The one-launch forms run 512-thread workgroups over a list of tickets.  A
+// ticket is the transforms of one length 2^L that one 512-thread pass holds,
+// in one of two tile shapes (the shape per length is measured at plan
+// creation):
+//   16384         : one transform, half-LDS form (32 values per thread);
+//   8192          : two transforms, half form on 256 threads each;
+//   <= 4096, E=16 : 8192 elements, 16 values per thread (radix-16 passes,
+//                   packed twiddle rows);
+//   <= 4096, E=8  : 4096 elements, 8 values per thread (radix-8 passes,
+//                   table twiddles), more threads per transform.
+// All within 64 KiB of LDS and 128 VGPRs: two workgroups per CU.  A ticket
+// code is log2 length + 16 * shape; the list is sorted by length, longest
+// first.
+constexpr int ticket_code(int l, int shape) { return l + 16 * shape; }
+constexpr int ticket_count(int l, int shape) {
+    return l == 14 ? 1 : l == 13 ? 2 : (shape ? 4096 : 8192) >> l;
+}
 
-template <int L, bool INV>
-__device__ __forceinline__ void queue_unit_body(const float2* in, float2* out, const float2* tw,
-                                                const int64_t* offs, int64_t first, int count, float2* lds) {
+// The next-ticket draw of lane 0 in the work-queue form: issued by draw()
+// before a ticket's data loads, written to its LDS slot by publish() right
+// after them (its latency hides behind the loads and its register is not
+// held through the passes).  A no-op in the one-shot form.
+struct TicketDraw {
+    int* counter;
+    int* slot;
+    int value;
+    __device__ __forceinline__ void draw() {
+        if (threadIdx.x == 0) value = atomicAdd(counter, 1);
+    }
+    __device__ __forceinline__ void publish() {
+        if (threadIdx.x == 0) *slot = value;
+    }
+};
+// The one-shot form draws nothing (no object on the stack: a kernel with a
+// private segment, even an unused one, is dispatched with scratch set up).
+struct NoDraw {
+    __device__ __forceinline__ void publish() {}
+};
+
+// One ticket: `count` transforms of length 2^L at sorted offsets
+// offs[first ...]; a slot past `count` transforms the ticket's last
+// transform again and stores nothing.
+template <int L, int SHAPE, bool INV, class Draw>
+__device__ __forceinline__ void ticket_body(const float2* in, float2* out, const float2* tw,
+                                            const int64_t* offs, int64_t first, int count, float2* lds,
+                                            Draw& td) {
     constexpr int N = 1 << L;
-    const int tid = fresh((int)threadIdx.x);  // index arithmetic redone per unit, not hoisted
-    if constexpr (L == 14) {
-        const int64_t off = offs[first];
-        half_transform<N, INV, true>(in + off, out + off, tw, lds, tid, true);
-    } else if constexpr (L == 13) {
-        const int h = tid >> 8, t = tid & 255;
-        const int64_t off = offs[first + (h < count ? h : count - 1)];
-        half_transform<N, INV, true>(in + off, out + off, tw, lds + h * (N / 2), t, h < count);
+    const int tid = fresh((int)threadIdx.x);  // index arithmetic redone per ticket, not hoisted
+    if constexpr (L >= 13) {
+        // half-LDS form (k_stockham_half's body): 16384 on all 512 threads,
+        // 8192 on each 256-thread half
+        constexpr int E = 32;
+        const int h = L == 14 ? 0 : tid >> 8;
+        const int t = L == 14 ? tid : tid & 255;
+        const bool active = h < count;
+        const int64_t off = offs[first + (active ? h : count - 1)];
+        float2 v[E];
+        load_column<N, E, INV, true>(v, in + off, t, true);
+        td.publish();
+        float2 w[E / 16 * 4];
+        half_bases<N, E, 1>(w, tw, t);
+        stockham_half<N, E, LdsSwz, 0>(v, lds + h * (N / 2), t, tw, w);
+        if (active) store_column<N, E, INV, true>(v, out + off, t);
     } else {
-        constexpr int T = N / 16;
+        constexpr int E = SHAPE ? 8 : 16;
+        constexpr int T = N / E;
         const int local = tid / T, t = tid % T;
-        // a slot past `count` transforms the unit's last transform again and
-        // stores nothing
         const int64_t off = offs[first + (local < count ? local : count - 1)];
-        float2 v[16];
-        load_column<N, 16, INV, true>(v, in + off, t, true);
-        PreBases<N, 16> pb;
-        pb.load_packed(tw, t);
-        stockham_regs<N, 16, LdsSwz, 0, true, true, false, 2>(v, lds + local * N, t, tw, pb.w);
-        if (local < count) store_column<N, 16, INV, true>(v, out + off, t);
+        float2 v[E];
+        load_column<N, E, INV, true>(v, in + off, t, true);
+        td.publish();
+        if constexpr (E == 16) {
+            PreBases<N, 16> pb;
+            pb.load_packed(tw, t);
+            stockham_regs<N, 16, LdsSwz, 0, false, true, false, 2>(v, lds + local * N, t, tw, pb.w);
+        } else {
+            stockham_regs<N, 8, LdsSwz, 0, false, true, false, 0>(v, lds + local * N, t, tw);
+        }
+        if (local < count) store_column<N, E, INV, true>(v, out + off, t);
     }
 }
 
-template <int L, bool INV>
-__device__ __forceinline__ void queue_ticket(const float2* in, float2* out, const float2* tw_all,
-                                             const int64_t* offs, int64_t first, int count, float2* lds) {
-    constexpr int U = queue_unit(L);
-    constexpr int kOff = table_off(L);
-    for (int b = 0; b < count; b += U) {  // count is uniform
-        if (b > 0) lds_barrier();        // the previous unit's LDS reads are done
-        queue_unit_body<L, INV>(in, out, tw_all + kOff, offs, first + b, count - b < U ? count - b : U, lds);
+template <bool INV, class Draw>
+__device__ __forceinline__ void run_ticket(const RaggedTile& d, const float2* in, float2* out,
+                                           const float2* tw_all, const int64_t* offs, float2* lds, Draw& td) {
+    const int64_t first = d.first;
+    const int count = d.count;
+    // the table offsets as constants: a plain call of the recursive
+    // table_off here is emitted as a device call (stack, full wait on entry)
+#define MIFFT_TK(L, S)                                                                                   \
+    case ticket_code(L, S): {                                                                            \
+        constexpr int off = table_off(L);                                                                \
+        ticket_body<L, S, INV, Draw>(in, out, tw_all + off, offs, first, count, lds, td);               \
+        break;                                                                                           \
     }
+    switch (d.log2n) {
+        MIFFT_TK(14, 0) MIFFT_TK(13, 0) MIFFT_TK(12, 0) MIFFT_TK(12, 1) MIFFT_TK(11, 0) MIFFT_TK(11, 1)
+        MIFFT_TK(10, 0) MIFFT_TK(10, 1) MIFFT_TK(9, 0) MIFFT_TK(9, 1) MIFFT_TK(8, 0) MIFFT_TK(8, 1)
+        MIFFT_TK(7, 0) MIFFT_TK(7, 1) MIFFT_TK(6, 0) MIFFT_TK(6, 1)
+        default: break;
+    }
+#undef MIFFT_TK
 }
 
+// Default form: workgroup b runs ticket b of [begin, end) and exits.  The
+// hardware dispatcher hands workgroups to CUs in order as slots free up, so
+// the longest lengths start first and the short tickets fill the end.
+template <bool INV>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_ragged_tickets(const float2* in, float2* out, const float2* __restrict__ tw_all,
+                 const RaggedTile* __restrict__ tickets, const int64_t* __restrict__ offs, int begin) {
+    __shared__ float2 lds[8192];
+    NoDraw td;
+    const RaggedTile d = tickets[begin + (int)blockIdx.x];
+    run_ticket<INV>(d, in, out, tw_all, offs, lds, td);
+}
+
+// Work-queue form: a persistent grid (the resident workgroups) takes the
+// tickets in list order from one device counter, each workgroup drawing its
+// next ticket while it runs the current one.  Between tickets it meets at an
+// LDS-only barrier.  A workgroup that draws past the end leaves; the last one
+// to leave (a second counter) sets both counters back to zero, so the queue
+// is ready for the next launch or graph replay without a memset.  (Measured
+// slower than the one-shot form: on this GPU one in-order counter, vmcnt,
+// covers loads and stores, so a persistent workgroup's next data wait also
+// waits for the previous ticket's 64-128 KiB of stores, while one-shot
+// workgroups overlap them with their successors' loads.)
 template <bool INV>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_ragged_queue(const float2* in, float2* out, const float2* __restrict__ tw_all,
                const RaggedTile* __restrict__ tickets, const int64_t* __restrict__ offs, int ntickets,
                int* queue) {
     __shared__ float2 lds[8192];
-    __shared__ int s_next;
-    if (threadIdx.x == 0) s_next = atomicAdd(&queue[0], 1);
-    __syncthreads();
-    int cur = __builtin_amdgcn_readfirstlane(s_next);
+    __shared__ int s_next[2];
+    if (threadIdx.x == 0) s_next[0] = atomicAdd(&queue[0], 1);
+    lds_barrier();
+    int cur = __builtin_amdgcn_readfirstlane(s_next[0]);
+    int par = 1;
     while (cur < ntickets) {  // uniform
-        int nxt = 0;
-        if (threadIdx.x == 0) nxt = atomicAdd(&queue[0], 1);  // in flight during this ticket
-        const RaggedTile d = tickets[cur];
-        const int64_t first = d.first;
-        const int count = d.count;
-        switch (d.log2n) {
-            case 14: queue_ticket<14, INV>(in, out, tw_all, offs, first, count, lds); break;
-            case 13: queue_ticket<13, INV>(in, out, tw_all, offs, first, count, lds); break;
-            case 12: queue_ticket<12, INV>(in, out, tw_all, offs, first, count, lds); break;
-            case 11: queue_ticket<11, INV>(in, out, tw_all, offs, first, count, lds); break;
-            case 10: queue_ticket<10, INV>(in, out, tw_all, offs, first, count, lds); break;
-            case 9: queue_ticket<9, INV>(in, out, tw_all, offs, first, count, lds); break;
-            case 8: queue_ticket<8, INV>(in, out, tw_all, offs, first, count, lds); break;
-            case 7: queue_ticket<7, INV>(in, out, tw_all, offs, first, count, lds); break;
-            default: queue_ticket<6, INV>(in, out, tw_all, offs, first, count, lds); break;
-        }
-        __syncthreads();  // every read of s_next and of this ticket's LDS is done
-        if (threadIdx.x == 0) s_next = nxt;
-        __syncthreads();
-        cur = __builtin_amdgcn_readfirstlane(s_next);
+        TicketDraw td{&queue[0], &s_next[par], 0};
+        td.draw();
+        run_ticket<INV>(tickets[cur], in, out, tw_all, offs, lds, td);
+        // LDS only: this ticket's stores stay in flight.  The slot written in
+        // this ticket is read after the barrier; the other slot, read after
+        // the previous barrier, is written again only in the next ticket.
+        lds_barrier();
+        cur = __builtin_amdgcn_readfirstlane(s_next[par]);
+        par ^= 1;
     }
     if (threadIdx.x == 0) {
         __threadfence();  // this workgroup's draws precede its exit count
@@ -303,6 +369,18 @@ k_ragged_queue(const float2* in, float2* out, const float2* __restrict__ tw_all,
             atomicExch(&queue[1], 0);
         }
     }
+}
+
+hipError_t launch_tickets(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tickets,
+                          const int64_t* offs, int begin, int end, bool inverse, hipStream_t st) {
+    if (end <= begin) return hipSuccess;
+    if (inverse)
+        hipLaunchKernelGGL(k_ragged_tickets<true>, dim3(end - begin), dim3(512), 0, st, in, out, tw_all, tickets,
+                           offs, begin);
+    else
+        hipLaunchKernelGGL(k_ragged_tickets<false>, dim3(end - begin), dim3(512), 0, st, in, out, tw_all, tickets,
+                           offs, begin);
+    return hipGetLastError();
 }
 
 hipError_t launch_queue(const float2* in, float2* out, const float2* tw_all, const RaggedTile* tickets,
@@ -367,10 +445,22 @@ mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool 
     const RaggedTile* tiles = static_cast<const RaggedTile*>(rp->d_tiles.ptr);
     const int64_t* offs = static_cast<const int64_t*>(rp->d_offsets.ptr);
     const float2* tw = static_cast<const float2*>(rp->d_twiddles.ptr);
+    const RaggedTile* tickets = static_cast<const RaggedTile*>(rp->d_tickets.ptr);
     if (mode & MIFFT_RAGGED_QUEUE)
-        return check_hip(launch_queue(in, out, tw, static_cast<const RaggedTile*>(rp->d_tickets.ptr), offs,
-                                      rp->ntickets, static_cast<int*>(rp->d_queue.ptr), inverse, st),
+        return check_hip(launch_queue(in, out, tw, tickets, offs, rp->ntickets, static_cast<int*>(rp->d_queue.ptr),
+                                      inverse, st),
                          "ragged fft (work queue)");
+    if (!(mode & MIFFT_RAGGED_SPLIT)) {
+        if (!(mode & MIFFT_RAGGED_PER_SIZE))
+            return check_hip(launch_tickets(in, out, tw, tickets, offs, 0, rp->ntickets, inverse, st), "ragged fft");
+        for (size_t q = 0; q + 1 < rp->size_ticket_begin.size(); ++q) {
+            const mifft_status r = check_hip(launch_tickets(in, out, tw, tickets, offs, rp->size_ticket_begin[q],
+                                                            rp->size_ticket_begin[q + 1], inverse, st),
+                                             "ragged fft (per size)");
+            if (r != MIFFT_OK) return r;
+        }
+        return MIFFT_OK;
+    }
     // lengths >= 4096: one launch each (their tiles hold one transform, so
     // the tile range is the sorted transform range)
     int small_begin = rp->ntiles;
@@ -380,24 +470,22 @@ mifft_status enqueue(mifft_ragged_plan* rp, const float2* in, float2* out, bool 
         if (l >= 12)
             r = check_hip(launch_ragged_one(l, in, out, tw, offs + rp->size_first[s], e - b, inverse, st),
                           "ragged fft");
-        else if (mode & MIFFT_RAGGED_PER_SIZE)
-            r = check_hip(launch_ragged_tiles(in, out, tw, tiles, offs, rp->ranges, b, e, inverse, st),
-                          "ragged fft (per size)");
         else if (b < small_begin)
             small_begin = b;
         if (r != MIFFT_OK) return r;
     }
-    if ((mode & MIFFT_RAGGED_PER_SIZE) == 0 && small_begin < rp->ntiles)
+    if (small_begin < rp->ntiles)
         return check_hip(launch_ragged_tiles(in, out, tw, tiles, offs, rp->ranges, small_begin, rp->ntiles, inverse,
                                              st),
                          "ragged fft");
     return MIFFT_OK;
 }
 
-// Work-queue tickets: the transforms of each length (sorted order, largest
-// length first) in runs of units_per_ticket[l - 6] * queue_unit(l).
+// Tickets of the one-launch forms: the transforms of each length (sorted
+// order, longest length first) in runs of ticket_count(l, shape), and the
+// first ticket of each length (plus the end).
 std::vector<RaggedTile> build_tickets(const int64_t* lengths, const std::vector<int64_t>& order, int64_t count,
-                                      const int* units_per_ticket) {
+                                      const int* shape, std::vector<int>* begins) {
     std::vector<RaggedTile> t;
     int64_t i = 0;
     while (i < count) {
@@ -406,17 +494,79 @@ std::vector<RaggedTile> build_tickets(const int64_t* lengths, const std::vector<
         while ((int64_t(1) << l) < n) ++l;
         int64_t j = i;
         while (j < count && lengths[order[j]] == n) ++j;
-        const int64_t per = (int64_t)queue_unit(l) * units_per_ticket[l - 6];
+        const int sh = l <= 12 ? shape[l - 6] : 0;
+        const int64_t per = ticket_count(l, sh);
+        begins->push_back((int)t.size());
         for (int64_t a = i; a < j; a += per) {
             RaggedTile d;
-            d.log2n = l;
+            d.log2n = ticket_code(l, sh);
             d.count = (int)std::min<int64_t>(per, j - a);
             d.first = a;
             t.push_back(d);
         }
         i = j;
     }
+    begins->push_back((int)t.size());
     return t;
+}
+
+// Tile shape of the one-launch forms for length 2^l (l <= 12): 16 or 8
+// values per thread, timed as ticket launches over 2^24 / n transforms and
+// recorded under "ragged fp32 c64 <n>".  Built-in choice: 8 values per
+// thread for 64 and 128 (where the uniform kernels' radix-8 form is the
+// faster one), 16 otherwise.
+int choose_shape(int l, const float2* tw_all, hipStream_t st) {
+    const int n = 1 << l;
+    const int fallback = l <= 7 ? 1 : 0;
+    if (!tuning_enabled()) return fallback;
+    const int64_t batch = (int64_t(1) << 24) / n;
+    std::vector<int64_t> offs(batch);
+    for (int64_t i = 0; i < batch; ++i) offs[i] = i * n;
+    std::vector<RaggedTile> tk[2];
+    for (int sh = 0; sh < 2; ++sh)
+        for (int64_t a = 0; a < batch; a += ticket_count(l, sh)) {
+            RaggedTile d;
+            d.log2n = ticket_code(l, sh);
+            d.count = (int)std::min<int64_t>(ticket_count(l, sh), batch - a);
+            d.first = a;
+            tk[sh].push_back(d);
+        }
+    DeviceBuffer d_offs, d_tk0, d_tk1;
+    if (d_offs.reserve(sizeof(int64_t) * batch) != MIFFT_OK ||
+        d_tk0.reserve(sizeof(RaggedTile) * tk[0].size()) != MIFFT_OK ||
+        d_tk1.reserve(sizeof(RaggedTile) * tk[1].size()) != MIFFT_OK ||
+        hipMemcpy(d_offs.ptr, offs.data(), sizeof(int64_t) * batch, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_tk0.ptr, tk[0].data(), sizeof(RaggedTile) * tk[0].size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_tk1.ptr, tk[1].data(), sizeof(RaggedTile) * tk[1].size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        d_offs.release();
+        d_tk0.release();
+        d_tk1.release();
+        return fallback;
+    }
+    const int64_t* o = static_cast<const int64_t*>(d_offs.ptr);
+    const RaggedTile* t0 = static_cast<const RaggedTile*>(d_tk0.ptr);
+    const RaggedTile* t1 = static_cast<const RaggedTile*>(d_tk1.ptr);
+    const int n0 = (int)tk[0].size(), n1 = (int)tk[1].size();
+    std::vector<Candidate> cand = {
+        {"ragged_tile_e16",
+         [=](const void* in, void* out, int64_t) {
+             return launch_tickets(static_cast<const float2*>(in), static_cast<float2*>(out), tw_all, t0, o, 0, n0,
+                                   false, st);
+         },
+         false},
+        {"ragged_tile_e8",
+         [=](const void* in, void* out, int64_t) {
+             return launch_tickets(static_cast<const float2*>(in), static_cast<float2*>(out), tw_all, t1, o, 0, n1,
+                                   false, st);
+         },
+         false}};
+    const int q = tuned_choice("ragged fp32 c64 " + std::to_string(n), n, 8, cand, st, "ragged");
+    (void)hipStreamSynchronize(st);
+    d_offs.release();
+    d_tk0.release();
+    d_tk1.release();
+    return q < 0 ? fallback : q;
 }
 
 void drop_graph(mifft_ragged_plan* rp) {
@@ -484,8 +634,6 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
     begins.push_back((int)tiles.size());
 
     mifft_ragged_plan* rp = new mifft_ragged_plan();
-    std::vector<RaggedTile> tickets = build_tickets(lengths, order, count, rp->units_per_ticket);
-    rp->ntickets = (int)tickets.size();
     rp->count = count;
     rp->extent = extent;
     rp->ntiles = (int)tiles.size();
@@ -502,13 +650,8 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
         mifft_ragged_plan_destroy(rp);
         return fail(MIFFT_ERR_NO_QUEUE, "Could not create HIP stream");
     }
-    if ((s = rp->d_tickets.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tickets.size()))) != MIFFT_OK ||
-        (s = rp->d_queue.reserve(2 * sizeof(int))) != MIFFT_OK ||
+    if ((s = rp->d_queue.reserve(2 * sizeof(int))) != MIFFT_OK ||
         (s = check_hip(hipMemset(rp->d_queue.ptr, 0, 2 * sizeof(int)), "hipMemset")) != MIFFT_OK ||
-        (!tickets.empty() &&
-         (s = check_hip(hipMemcpy(rp->d_tickets.ptr, tickets.data(), sizeof(RaggedTile) * tickets.size(),
-                                  hipMemcpyHostToDevice),
-                        "hipMemcpy")) != MIFFT_OK) ||
         (s = rp->d_tiles.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tiles.size()))) != MIFFT_OK ||
         (s = rp->d_offsets.reserve(sizeof(int64_t) * std::max<int64_t>(1, count))) != MIFFT_OK ||
         (s = rp->d_twiddles.reserve(sizeof(float2) * ragged_twiddles_len())) != MIFFT_OK) {
@@ -525,6 +668,19 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
         (s = check_hip(launch_ragged_twiddles(static_cast<float2*>(rp->d_twiddles.ptr), rp->stream),
                        "twiddles")) != MIFFT_OK ||
         (s = check_hip(hipStreamSynchronize(rp->stream), "hipStreamSynchronize")) != MIFFT_OK) {
+        mifft_ragged_plan_destroy(rp);
+        return s;
+    }
+    // tile shape per length present (measured once per length and process)
+    for (int l : log2s)
+        if (l <= 12) rp->shape[l - 6] = choose_shape(l, static_cast<const float2*>(rp->d_twiddles.ptr), rp->stream);
+    std::vector<RaggedTile> tickets = build_tickets(lengths, order, count, rp->shape, &rp->size_ticket_begin);
+    rp->ntickets = (int)tickets.size();
+    if ((s = rp->d_tickets.reserve(sizeof(RaggedTile) * std::max<size_t>(1, tickets.size()))) != MIFFT_OK ||
+        (!tickets.empty() &&
+         (s = check_hip(hipMemcpy(rp->d_tickets.ptr, tickets.data(), sizeof(RaggedTile) * tickets.size(),
+                                  hipMemcpyHostToDevice),
+                        "hipMemcpy")) != MIFFT_OK)) {
         mifft_ragged_plan_destroy(rp);
         return s;
     }
@@ -551,8 +707,8 @@ int64_t mifft_ragged_elements(const mifft_ragged_plan* rp) { return rp ? rp->ele
 int64_t mifft_ragged_launches(const mifft_ragged_plan* rp, int mode) {
     if (!rp) return -1;
     if (rp->ntiles == 0) return 0;
-    if (mode & MIFFT_RAGGED_QUEUE) return 1;
     if (mode & MIFFT_RAGGED_PER_SIZE) return (int64_t)rp->size_tile_begin.size() - 1;
+    if (!(mode & MIFFT_RAGGED_SPLIT)) return 1;  // tickets or work queue
     int64_t n = 0;
     bool small = false;
     for (int l : rp->size_log2) {
@@ -567,10 +723,12 @@ mifft_status mifft_ragged_execute_device(mifft_ragged_plan* rp, const float* in,
     if (!rp) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null plan");
     if (dir != MIFFT_FORWARD && dir != MIFFT_INVERSE)
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "direction must be MIFFT_FORWARD or MIFFT_INVERSE");
-    if (mode & ~(MIFFT_RAGGED_PER_SIZE | MIFFT_RAGGED_GRAPH | MIFFT_RAGGED_QUEUE))
+    if (mode & ~(MIFFT_RAGGED_PER_SIZE | MIFFT_RAGGED_GRAPH | MIFFT_RAGGED_QUEUE | MIFFT_RAGGED_SPLIT))
         return fail(MIFFT_ERR_INVALID_ARGUMENT, "unknown mode bits");
-    if ((mode & MIFFT_RAGGED_PER_SIZE) && (mode & MIFFT_RAGGED_QUEUE))
-        return fail(MIFFT_ERR_INVALID_ARGUMENT, "MIFFT_RAGGED_PER_SIZE and MIFFT_RAGGED_QUEUE exclude each other");
+    if (((mode & MIFFT_RAGGED_PER_SIZE) != 0) + ((mode & MIFFT_RAGGED_QUEUE) != 0) +
+            ((mode & MIFFT_RAGGED_SPLIT) != 0) > 1)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT,
+                    "MIFFT_RAGGED_PER_SIZE, MIFFT_RAGGED_QUEUE and MIFFT_RAGGED_SPLIT exclude each other");
     if (rp->count == 0) return MIFFT_OK;
     if (!in || !out) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null buffer");
     hipStream_t st = static_cast<hipStream_t>(stream);

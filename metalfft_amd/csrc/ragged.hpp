@@ -6,7 +6,8 @@
 
 namespace mifft {
 // One workgroup-iteration: `count` transforms of length 2^log2n whose
-// offsets are offs[first .. first + count).
+// offsets are offs[first .. first + count).  In a one-launch ticket, log2n
+// holds the ticket code (log2 length + 16 * tile shape).
 struct RaggedTile {
     int32_t log2n;
     int32_t count;
@@ -30,10 +31,13 @@ struct mifft_ragged_plan {
     mifft::RaggedRanges ranges = {};   // the same ranges indexed by log2 length
     std::vector<int64_t> size_first;   // sorted index of each length's first transform
     mifft::DeviceBuffer d_tiles, d_offsets, d_twiddles;
-    // work-queue form: tickets (length, count, first) sorted largest length
-    // first, and the two device counters (next ticket, workgroups done)
+    // one-launch forms: tickets (code, count, first) sorted longest length
+    // first, the ticket range of each length (aligned with size_log2), the
+    // tile shape per length (0: 16 values per thread, 1: 8; by log2 - 6) and
+    // the work-queue form's two device counters (next ticket, workgroups done)
     int ntickets = 0;
-    int units_per_ticket[9] = {1, 1, 1, 1, 1, 1, 1, 1, 1};  // by log2 length - 6
+    std::vector<int> size_ticket_begin;
+    int shape[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     mifft::DeviceBuffer d_tickets, d_queue;
     hipStream_t stream = nullptr;  // capture stream
     hipGraph_t graph = nullptr;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -48,6 +49,21 @@ struct DeviceBuffer {
 // Kernel variant per size: picked by the autotuner (timed on first use and
 // cached on disk) or the built-in default.
 const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* tw);
+// Autotuner core.  One timing candidate: a launch of `batch` transforms of
+// the tuned length from in to out (device buffers of 2^24 / n transforms).
+struct Candidate {
+    const char* name;
+    std::function<hipError_t(const void* in, void* out, int64_t batch)> run;
+    bool prefer;  // stateless one-shot form, kept when within 3 % of the fastest
+};
+// The recorded choice for `key` ("<kind> <precision> <io> <n>"), or the
+// fastest of `cand` timed now on `elem_bytes`-byte elements (recorded);
+// the candidate index, or -1 if none ran.
+int tuned_choice(const std::string& key, int n, size_t elem_bytes, const std::vector<Candidate>& cand,
+                 hipStream_t stream, const char* what);
+// False under MIFFT_AUTOTUNE=0 (built-in defaults, nothing timed).
+bool tuning_enabled();
+
 // Lengths above 16384: the fastest of the single-workgroup variants of n
 // (tw: their table, or null) and the four-step variants (fs_tw, or null);
 // exactly one of *variant / *fourstep is set.

@@ -1,8 +1,10 @@
-"""Ragged batches (lengths 64 ... 16384 mixed): one persistent launch fed by
-a device work queue (tickets of one length, longest first), one launch per
-length of 4096 and longer plus one persistent launch for the shorter ones,
-or one launch per size, optionally replayed from a hipGraph, against FP64
-per transform; every form gives the same bits."""
+"""Ragged batches (lengths 64 ... 16384 mixed) against FP64 per transform,
+in every launch structure: one launch of length-sorted tickets (default),
+one launch per length, one persistent launch fed by a device ticket counter
+(work queue), and the split form (one launch per length of 4096 and longer
+plus one persistent launch of short tiles); optionally replayed from a
+hipGraph.  The ticket forms (default, per size, queue) run the same code per
+ticket and give the same bits."""
 import numpy as np
 import pytest
 
@@ -33,15 +35,20 @@ def check_all(oracle, lengths, x, y, inverse=False, tol=1e-6, stride=1):
     assert worst < tol, worst
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+MODES = [0, 1, 2, 3, 4, 6, 8, 10]
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_ragged_small(gpu, oracle, mode):
     import torch
     lengths = ragged_lengths(300, 1)
     x = ragged_data(lengths, 1)
     rp = gpu.RaggedFFT(lengths)
     assert rp.extent == x.size and rp.elements == x.size
-    classes = len({int(n) if n >= 4096 else 0 for n in lengths})  # one launch per length >= 4096
-    assert rp.launches(mode) == (classes if mode & 1 == 0 else len(set(lengths.tolist())))
+    split = len({int(n) if n >= 4096 else 0 for n in lengths})  # one launch per length >= 4096
+    expect = (len(set(lengths.tolist())) if mode & gpu.RAGGED_PER_SIZE else
+              split if mode & gpu.RAGGED_SPLIT else 1)
+    assert rp.launches(mode) == expect
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
     rp.execute(xd, yd, mode=mode)
@@ -56,17 +63,16 @@ def test_ragged_small(gpu, oracle, mode):
 @pytest.mark.parametrize("lengths", [[64, 4096, 128, 1024] * 40, [8192, 64] * 30, [16384] * 5,
                                      [16384, 8192] * 7, [2048] * 3 + [8192]])
 def test_ragged_tile_classes(gpu, oracle, lengths):
-    """Plans with lengths of 4096 and longer (one launch each), shorter
-    lengths (one persistent launch of 4096-element tiles), or both;
-    persistent and per size."""
+    """Plans with lengths of 4096 and longer, shorter lengths, or both, in
+    every launch structure."""
     import torch
     lengths = np.array(lengths, np.int64)
     x = ragged_data(lengths, 5)
     rp = gpu.RaggedFFT(lengths)
-    classes = len({int(n) if n >= 4096 else 0 for n in lengths})  # one launch per length >= 4096
-    assert rp.launches() == classes
+    split = len({int(n) if n >= 4096 else 0 for n in lengths})  # one launch per length >= 4096
+    assert rp.launches() == 1 and rp.launches(gpu.RAGGED_SPLIT) == split
     xd = torch.from_numpy(x).cuda()
-    for mode in (0, 1):
+    for mode in (gpu.RAGGED_DEFAULT, gpu.RAGGED_PER_SIZE, gpu.RAGGED_QUEUE, gpu.RAGGED_SPLIT):
         yd = torch.empty_like(xd)
         rp.execute(xd, yd, mode=mode)
         torch.cuda.synchronize()
@@ -131,10 +137,12 @@ def test_graph_replays_match_eager(gpu, oracle):
 @pytest.mark.parametrize("lengths", [[64, 4096, 128, 1024] * 40, [8192, 64] * 31, [16384] * 5,
                                      [16384, 8192] * 7 + [8192], [2048] * 3 + [8192], [64] * 129,
                                      [4096] * 3, [256, 512, 1024, 2048] * 50])
-def test_queue_matches_other_forms(gpu, oracle, lengths):
-    """The work-queue launch equals the per-length launches bit for bit
-    (same transform code per length), including partial tickets (an odd
-    number of 8192 or 4096 transforms, a short last tile), and is one launch."""
+def test_ticket_forms_agree(gpu, oracle, lengths):
+    """The default ticket launch, one launch per length and the work queue run
+    the same code per ticket: equal bit for bit, including partial tickets (an
+    odd number of 8192 or 4096 transforms, a short last tile); the split form
+    within FP32 rounding.  The queue counters reset themselves: repeated
+    launches give the same bits."""
     import torch
     lengths = np.array(lengths, np.int64)
     x = ragged_data(lengths, 6)
@@ -142,16 +150,23 @@ def test_queue_matches_other_forms(gpu, oracle, lengths):
     assert rp.launches(gpu.RAGGED_QUEUE) == 1
     xd = torch.from_numpy(x).cuda()
     ref = torch.empty_like(xd)
-    rp.execute(xd, ref, mode=gpu.RAGGED_PERSISTENT)
+    rp.execute(xd, ref)
+    per = torch.empty_like(xd)
+    rp.execute(xd, per, mode=gpu.RAGGED_PER_SIZE)
+    spl = torch.empty_like(xd)
+    rp.execute(xd, spl, mode=gpu.RAGGED_SPLIT)
+    torch.cuda.synchronize()
+    assert torch.equal(per, ref)
+    assert oracle.rel_l2(spl.cpu().numpy(), ref.cpu().numpy()) < 1e-6
     q = torch.empty_like(xd)
-    for rep in range(3):  # the queue counters reset themselves between launches
+    for rep in range(3):
         q.fill_(float("nan"))
         rp.execute(xd, q, mode=gpu.RAGGED_QUEUE)
         torch.cuda.synchronize()
         assert torch.equal(q, ref), rep
-    check_all(oracle, lengths, x, q.cpu().numpy())
+    check_all(oracle, lengths, x, ref.cpu().numpy())
     back = torch.empty_like(xd)
-    rp.execute(q, back, direction=gpu.INVERSE, mode=gpu.RAGGED_QUEUE)
+    rp.execute(ref, back, direction=gpu.INVERSE)
     torch.cuda.synchronize()
     assert oracle.rel_l2(back.cpu().numpy(), x) < 1e-6
 
@@ -186,7 +201,8 @@ def test_queue_graph_and_in_place(gpu, oracle):
 
 def test_config2_ten_thousand(gpu, oracle):
     """10,000 transforms, lengths log-uniform over {64..16384} (seed 1), N(0,1)
-    values: forward + inverse, persistent vs per-size; FP64 on a sample."""
+    values: forward + inverse in one launch, against the other launch
+    structures; FP64 on a sample."""
     import torch
     lengths = ragged_lengths(10000, 1)
     rng = np.random.default_rng(1)
@@ -197,12 +213,16 @@ def test_config2_ten_thousand(gpu, oracle):
     y1 = torch.empty_like(xd)
     y2 = torch.empty_like(xd)
     y3 = torch.empty_like(xd)
-    rp.execute(xd, y1, mode=gpu.RAGGED_PERSISTENT)
+    y4 = torch.empty_like(xd)
+    assert rp.launches() == 1
+    rp.execute(xd, y1)
     rp.execute(xd, y2, mode=gpu.RAGGED_PER_SIZE)
     rp.execute(xd, y3, mode=gpu.RAGGED_QUEUE)
+    rp.execute(xd, y4, mode=gpu.RAGGED_SPLIT | gpu.RAGGED_GRAPH)
     torch.cuda.synchronize()
     assert torch.equal(y1, y2)  # same math, different launch structure
     assert torch.equal(y1, y3)
+    assert oracle.rel_l2(y4.cpu().numpy(), y1.cpu().numpy()) < 1e-6
     check_all(oracle, lengths, x, y1.cpu().numpy(), stride=97)
     z = torch.empty_like(xd)
     rp.execute(y1, z, direction=gpu.INVERSE)
