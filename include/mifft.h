@@ -236,6 +236,27 @@ MIFFT_API mifft_status mifft_convolver_backward_device(mifft_convolver *cv, cons
                                                        const float *grad_out, float *grad_signal,
                                                        float *grad_taps, void *stream);
 
+/* Adjoint that allocates nothing.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as mifft_convolver_backward_device, but the tap
+ * partials live in one fixed 16 MiB array per GPU (a static device
+ * variable) instead of convolver-owned memory grown on demand.  When the
+ * allocating form's partials fit there -- (slots + 2 ceil(slots / 64)) * K
+ * floats, slots <= 2048 workgroups x transforms per workgroup, e.g. 8.6 MB
+ * for 512 x 65536 samples, K = 1023, fft size 4096 -- the results are bit
+ * for bit the same; otherwise the work is split over fewer workgroups (the
+ * same arithmetic per block pair, the tap sum in another order, within
+ * FP32 rounding).  A split that cannot fit even with one workgroup returns
+ * MIFFT_ERR_INVALID_ARGUMENT with the bytes needed (expected = the array's
+ * size, got = the need).  Calls on one GPU share the array: calls that
+ * overlap in time on one GPU race. */
+MIFFT_API mifft_status mifft_convolver_backward_device_noalloc(const mifft_convolver *cv,
+                                                               const float *signals, int64_t samples,
+                                                               int64_t count, const float *grad_out,
+                                                               float *grad_signal, float *grad_taps,
+                                                               void *stream);
+
 /* ------------------------------------------------------------ correlator */
 typedef struct mifft_correlator mifft_correlator;
 

@@ -90,6 +90,8 @@ _SIGNATURES = [
     ("mifft_convolver_set_taps_device", c_int, [c_void_p, c_void_p, c_void_p]),
     ("mifft_convolver_backward_device", c_int,
      [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("mifft_convolver_backward_device_noalloc", c_int,
+     [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("mifft_ragged_plan_create", c_int, [i64p, i64p, c_int64, ctypes.POINTER(c_void_p)]),
     ("mifft_ragged_plan_destroy", c_int, [c_void_p]),
     ("mifft_ragged_extent", c_int64, [c_void_p]),

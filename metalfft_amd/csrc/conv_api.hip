@@ -5,6 +5,7 @@
 // Sources/MetalFFT/DSP/Correlate.swift:7-62.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "convolve.hpp"
@@ -155,6 +156,73 @@ mifft_status mifft_convolver_backward_device(mifft_convolver* cv, const float* s
         return zero_floats(grad_taps, cv->K, st);
     }
     return convolver_backward(cv, signals, samples, count, grad_out, grad_signal, grad_taps, st);
+}
+
+// --------------------------------------------- adjoint without allocation
+// Callers that cannot allocate (code under hipGraph capture, tight serving
+// loops) get the adjoint with its tap partials in this file-scope array
+// instead of convolver-owned memory that grows on demand.  Each GPU holds
+// its own copy for the life of the process; host code uses the address
+// hipGetSymbolAddress returns for the current device, never the array's
+// name.  Every call on one GPU shares it: calls that overlap in time on one
+// GPU (two streams, two threads) race.
+constexpr size_t kNoAllocScratchFloats = (size_t(16) << 20) / sizeof(float);  // 16 MiB
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_dev[kMaxDevices] = {};
+
+// Address of the current device's scratch array, looked up once per device.
+mifft_status noalloc_scratch(float** out) {
+    int dev = 0;
+    mifft_status s = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (s != MIFFT_OK) return s;
+    if (dev < 0 || dev >= kMaxDevices) return fail(MIFFT_ERR_NO_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        if ((s = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) != MIFFT_OK)
+            return s;
+        g_scratch_dev[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return MIFFT_OK;
+}
+}  // namespace
+
+mifft_status mifft_convolver_backward_device_noalloc(const mifft_convolver* cv, const float* signals,
+                                                     int64_t samples, int64_t count, const float* grad_out,
+                                                     float* grad_signal, float* grad_taps, void* stream) {
+    if (!cv) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null convolver");
+    if (cv->precision != MIFFT_CONV_FP32)
+        return fail(MIFFT_ERR_KERNEL_NOT_FOUND, "HIP kernel not found: adjoint is FP32 only");
+    if (samples < 0 || count < 0) return fail(MIFFT_ERR_INVALID_ARGUMENT, "negative size");
+    if (!grad_taps) return fail(MIFFT_ERR_INVALID_ARGUMENT, "null grad_taps");
+    if (count > 0 && samples > 0 && (!signals || !grad_out || !grad_signal))
+        return fail(MIFFT_ERR_INVALID_ARGUMENT, "null buffer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (count == 0 || samples == 0) return zero_floats(grad_taps, cv->K, st);
+    float* part = nullptr;
+    mifft_status s = noalloc_scratch(&part);
+    if (s != MIFFT_OK) return s;
+    // the allocating entry point's work split when its partials fit (same
+    // bits), else fewer, longer-running workgroups
+    const int64_t cap = (int64_t)kNoAllocScratchFloats;
+    int64_t target = kAdjointWorkgroups;
+    AdjointGeom a = adjoint_geometry(cv, samples, count, target);
+    while (adjoint_scratch_floats(a, cv->K) > cap && target > 1) {
+        target = target * 3 / 4;
+        a = adjoint_geometry(cv, samples, count, target);
+    }
+    const int64_t need = adjoint_scratch_floats(a, cv->K);
+    if (need > cap)
+        return fail(MIFFT_ERR_INVALID_ARGUMENT,
+                    "no-allocation adjoint: tap partials need " + std::to_string(need * 4) +
+                        " bytes, the fixed scratch holds " + std::to_string(cap * 4),
+                    cap * 4, need * 4);
+    return convolver_backward_with(cv, signals, samples, count, grad_out, grad_signal, grad_taps, part, a, st);
 }
 
 // ------------------------------------------------------------ correlator

@@ -723,35 +723,49 @@ mifft_status convolver_forward(const mifft_convolver* cv, const float* x, int64_
                      "fused convolve");
 }
 
-mifft_status convolver_backward(mifft_convolver* cv, const float* x, int64_t S, int64_t count,
-                                 const float* gy, float* gx, float* gh, hipStream_t st) {
+AdjointGeom adjoint_geometry(const mifft_convolver* cv, int64_t S, int64_t count, int64_t target_wg) {
     ConvGeom g = geometry(cv, S, count);
-    const int64_t nbx = (S + g.hop - 1) / g.hop;
-    const int64_t npx = (nbx + 1) / 2;
+    AdjointGeom a;
+    a.nbx = (S + g.hop - 1) / g.hop;
+    a.npx = (a.nbx + 1) / 2;
     const int tpw = conv_tpw_for((int)cv->n);
-    const int64_t items = count * npx;
-    // ~2048 workgroups: enough to fill 256 CUs, few partial rows to reduce
-    const int64_t target_wg = 2048;
-    int64_t ppw = (items + target_wg * tpw - 1) / (target_wg * tpw);
-    if (ppw < 1) ppw = 1;
-    const int64_t blocks = (items + ppw * tpw - 1) / (ppw * tpw);
-    const int64_t slots = blocks * tpw;
-    // partials, then the 64-row sums of them (tree reduction scratch)
-    const int64_t level1 = (slots + 63) / 64;
-    mifft_status s = cv->partial.reserve(sizeof(float) * (size_t)(slots + 2 * level1) * (size_t)cv->K);
-    if (s != MIFFT_OK) return s;
-    float* part = static_cast<float*>(cv->partial.ptr);
-    if ((s = check_hip(conv_bwd((int)cv->n, x, gy, gx, part, slots,
+    const int64_t items = count * a.npx;
+    a.ppw = (items + target_wg * tpw - 1) / (target_wg * tpw);
+    if (a.ppw < 1) a.ppw = 1;
+    const int64_t blocks = (items + a.ppw * tpw - 1) / (a.ppw * tpw);
+    a.slots = blocks * tpw;
+    a.level1 = (a.slots + 63) / 64;
+    return a;
+}
+
+int64_t adjoint_scratch_floats(const AdjointGeom& a, int64_t K) { return (a.slots + 2 * a.level1) * K; }
+
+mifft_status convolver_backward_with(const mifft_convolver* cv, const float* x, int64_t S, int64_t count,
+                                     const float* gy, float* gx, float* gh, float* part, const AdjointGeom& a,
+                                     hipStream_t st) {
+    ConvGeom g = geometry(cv, S, count);
+    mifft_status s;
+    if ((s = check_hip(conv_bwd((int)cv->n, x, gy, gx, part, a.slots,
                                 static_cast<const float2*>(cv->spectrum.ptr), cv->plan->twiddles, g,
-                                nbx, npx, (int)ppw, st),
+                                a.nbx, a.npx, (int)a.ppw, st),
                        "fused convolve adjoint")) != MIFFT_OK)
         return s;
     // grad_h = column sums of the slot partials, ping-pong between two
     // scratch areas after `part`
     const int K = (int)cv->K;
-    return check_hip(sum_rows(part, slots, K, part + slots * (int64_t)K, part + (slots + level1) * (int64_t)K,
-                              gh, st),
+    return check_hip(sum_rows(part, a.slots, K, part + a.slots * (int64_t)K,
+                              part + (a.slots + a.level1) * (int64_t)K, gh, st),
                      "tap gradient reduction");
+}
+
+mifft_status convolver_backward(mifft_convolver* cv, const float* x, int64_t S, int64_t count,
+                                 const float* gy, float* gx, float* gh, hipStream_t st) {
+    // ~2048 workgroups: enough to fill 256 CUs, few partial rows to reduce
+    const AdjointGeom a = adjoint_geometry(cv, S, count, kAdjointWorkgroups);
+    // partials, then the 64-row sums of them (tree reduction scratch)
+    mifft_status s = cv->partial.reserve(sizeof(float) * (size_t)adjoint_scratch_floats(a, cv->K));
+    if (s != MIFFT_OK) return s;
+    return convolver_backward_with(cv, x, S, count, gy, gx, gh, static_cast<float*>(cv->partial.ptr), a, st);
 }
 
 hipError_t sum_rows(const float* in, int64_t rows, int cols, float* s0, float* s1, float* out,

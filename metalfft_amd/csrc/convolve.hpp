@@ -24,6 +24,21 @@ mifft_status convolver_forward(const mifft_convolver* cv, const float* x, int64_
 // gx[count][S], gh[K] from x[count][S] and gy[count][S + K - 1].
 mifft_status convolver_backward(mifft_convolver* cv, const float* x, int64_t S, int64_t count,
                                 const float* gy, float* gx, float* gh, hipStream_t st);
+// Work split of the adjoint's tap-gradient kernel: block pairs per signal,
+// pairs per workgroup, one partial row of K floats per slot (transform
+// slot of a workgroup) and the first level of the 64-row sum tree.
+struct AdjointGeom {
+    int64_t nbx, npx, ppw, slots, level1;
+};
+constexpr int64_t kAdjointWorkgroups = 2048;  // enough to fill 256 CUs
+AdjointGeom adjoint_geometry(const mifft_convolver* cv, int64_t S, int64_t count, int64_t target_wg);
+// Scratch floats the split needs: the partial rows and two tree levels.
+int64_t adjoint_scratch_floats(const AdjointGeom& a, int64_t K);
+// The adjoint with its partials in `part` (adjoint_scratch_floats(a, K)
+// floats, every one written before it is read).
+mifft_status convolver_backward_with(const mifft_convolver* cv, const float* x, int64_t S, int64_t count,
+                                     const float* gy, float* gx, float* gh, float* part, const AdjointGeom& a,
+                                     hipStream_t st);
 // r[count][N] <- circular cross-correlation of a[count][na], b[count][nb]
 // (each truncated or zero-padded to N = plan size).
 mifft_status correlate_device(const mifft_plan* plan, const float* a, int64_t na, const float* b,

@@ -96,6 +96,19 @@ class Convolver:
             _dev_ptr(grad_signal, 4 * samples * count, F32, "grad_signal"),
             _dev_ptr(grad_taps, 4 * self.kernelLen, F32, "grad_taps"), _stream_handle(stream)))
 
+    def backward_device_noalloc(self, signals, grad_out, grad_signal, grad_taps, samples: int,
+                                count: int, stream=None) -> None:
+        """backward_device without any device allocation: the tap partials
+        live in the library's fixed per-GPU scratch array (calls on one GPU
+        must not overlap)."""
+        samples, count = _count(samples, "samples"), _count(count, "count")
+        out_len = samples + self.kernelLen - 1
+        _lib.check(_lib.load().mifft_convolver_backward_device_noalloc(
+            self._cv, _dev_ptr(signals, 4 * samples * count, F32, "signals"), samples, count,
+            _dev_ptr(grad_out, 4 * out_len * count, F32, "grad_out"),
+            _dev_ptr(grad_signal, 4 * samples * count, F32, "grad_signal"),
+            _dev_ptr(grad_taps, 4 * self.kernelLen, F32, "grad_taps"), _stream_handle(stream)))
+
     def close(self) -> None:
         if getattr(self, "_cv", None) and self._cv.value:
             _lib.load().mifft_convolver_destroy(self._cv)
