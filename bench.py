@@ -44,6 +44,15 @@ def parse():
                    help="skip the MFMA / ragged / convolve / SAR measurements")
     p.add_argument("--only", default=None,
                    help="comma-separated extras to run (default: all), e.g. stft_welch_2p24")
+    p.add_argument("--replays", type=int, default=31,
+                   help="timed graph replays of the K launches (back to back); the median "
+                        "is reported")
+    p.add_argument("--settle", type=float, default=3.0,
+                   help="at most this many seconds of untimed replays before timing, until "
+                        "the replay time stops falling")
+    p.add_argument("--probe", default=None, choices=sorted(PROBES),
+                   help="only run a few launches of one kernel's workload (for rocprofv3 "
+                        "--pmc counter runs) and exit")
     p.add_argument("--traffic-probe", action="store_true",
                    help="only run 4 launches of a plain 128 MiB copy (elementwise, known "
                         "bytes) and 4 of the FFT step, for rocprofv3 --pmc FETCH_SIZE / "
@@ -87,12 +96,20 @@ def time_gpu(fn, steps, warmup, torch):
     return e0.elapsed_time(e1) / steps, wall
 
 
-def time_graph(fn, steps, warmup, torch):
+def time_graph(fn, steps, warmup, torch, reps=1, settle_s=0.0):
     """The same `steps` launches captured once into a hipGraph and replayed:
     the GPU runs them back to back with no host launch jitter between them.
     Warm-up launches run eagerly first (plans and occupancy queries settle
-    outside the capture), then two untimed replays; HIP events on the replay
-    stream bracket the timed replay."""
+    outside the capture).  With settle_s > 0, untimed replays then run back
+    to back in chunks of about 2 ms of GPU time until the per-replay time of
+    a chunk stops falling (no new minimum, by 0.3 %, in three chunks) or
+    settle_s seconds have passed: the clocks reach the state a long run
+    holds (a 20-launch replay right after start-up reads several percent
+    slow).  Then `reps` replays are enqueued back to back, each bracketed by
+    its own pair of HIP events on the replay stream, with one
+    synchronisation after the last: the result is the median replay time
+    per launch.  Returns (ms per launch, wall seconds of the timed replays,
+    graph, [ms per launch of each timed replay], settle replays run)."""
     for i in range(warmup):
         fn(i, None)
     torch.cuda.synchronize()
@@ -101,20 +118,36 @@ def time_graph(fn, steps, warmup, torch):
         capture = torch.cuda.current_stream().cuda_stream
         for i in range(steps):
             fn(i, capture)
-    graph.replay()
-    graph.replay()
-    torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    with no_gc():
-        t0 = time.perf_counter()
-        e0.record(stream)
-        graph.replay()
-        e1.record(stream)
+
+    def timed(count):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(count)]
+        for e0, e1 in ev:
+            e0.record(stream)
+            graph.replay()
+            e1.record(stream)
         torch.cuda.synchronize()
+        return [e0.elapsed_time(e1) for e0, e1 in ev]
+
+    settled = 0
+    with no_gc():
+        one = min(timed(2))
+        if settle_s > 0:
+            chunk = max(1, int(2.0 / max(one, 1e-3)))  # ~2 ms of GPU time per chunk
+            best, since, t_start = float("inf"), 0, time.perf_counter()
+            while since < 3 and time.perf_counter() - t_start < settle_s:
+                t = sorted(timed(chunk))[chunk // 2]
+                settled += chunk
+                if t < best * 0.997:
+                    best, since = t, 0
+                else:
+                    since += 1
+        t0 = time.perf_counter()
+        times = timed(reps)
         wall = time.perf_counter() - t0
-    return e0.elapsed_time(e1) / steps, wall, graph
+    per = sorted(t / steps for t in times)
+    return per[len(per) // 2], wall, graph, [t / steps for t in times], settled
 
 
 def copy_ceiling(sets, steps, torch):
@@ -132,10 +165,11 @@ def copy_ceiling(sets, steps, torch):
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _lib.check(L.mifft_copy_device(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
                                        ctypes.c_int64(nbytes), ctypes.c_void_p(s)))
-    ms, _, graph = time_graph(fn, steps, 10, torch)
+    ms, _, graph, _, _ = time_graph(fn, steps, 10, torch, reps=3, settle_s=1.0)
     del graph
     return {"gbs": 2 * nbytes / (ms * 1e-3) / 1e9, "ms_per_copy": ms, "bytes_per_copy": 2 * nbytes,
-            "kernel": "mifft_copy_device, 4 rotating 128 MiB sets"}
+            "kernel": "mifft_copy_device, 4 rotating 128 MiB sets",
+            "method": "median of 3 back-to-back replays of the K launches after settling"}
 
 
 def cpu_baseline(n, batch, torch_seed_data):
@@ -451,7 +485,7 @@ def extra_sizes(m, torch):
             _lib.check(L.mifft_execute_device(f._plan, ctypes.c_void_p(src.data_ptr()),
                                               ctypes.c_void_p(dst.data_ptr()), ctypes.c_int64(batch),
                                               ctypes.c_int(m.FORWARD), st))
-        ms, _, graph = time_graph(step, 200, 8, torch)
+        ms, _, graph, _, _ = time_graph(step, 200, 8, torch)
         del graph
         gbs = 16 * n * batch / (ms * 1e-3) / 1e9
         out[str(n)] = {"kernel": f.kernel_name, "us": ms * 1e3, "transforms_per_s": batch / (ms * 1e-3),
@@ -508,13 +542,78 @@ def extra_stft(m, torch, s=1 << 24, n=1024, hop=256):
             "welch_frames_per_s": f / (ms_w * 1e-3)}
 
 
-def load_traffic(name):
+def load_traffic(name, kernel):
+    """HBM bytes per launch from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary, only if it was measured on the kernel variant now timed;
+    (bytes or None, note)."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            d = json.load(f)
     except Exception:
-        return None
+        return None, f"no traffic summary {name}"
+    measured = d.get("variant") or d.get("kernel", "").split(" ")[0]
+    if measured != kernel:
+        return None, f"traffic in {name} was measured on {measured}, the timed kernel is {kernel}"
+    return d.get("hbm_bytes_per_launch"), f"{name} (measured on {measured})"
+
+
+# --probe workloads: a few launches each, nothing timed (rocprofv3 --pmc runs)
+PROBES = {
+    "fft4096": "batched FFT N=4096, batch 4096, complex64 (the headline kernel)",
+    "fft8192": "batched FFT N=8192, batch 2048, complex64",
+    "fft16384": "batched FFT N=16384, batch 1024, complex64",
+    "mfma_fp16split_c64": "DFT-as-GEMM FP16 hi/lo, N=4096, batch 4096, complex64 I/O",
+    "mfma_fp16split_c32h": "DFT-as-GEMM FP16 hi/lo, N=4096, batch 4096, complex-FP16 I/O",
+    "sar_fp16": "range-Doppler 4096 x 4096 complex FP16, RCMC on, FP32 arithmetic",
+    "convolve": "fused convolution 512 x 65536, K = 1023, N = 4096, forward + adjoint",
+}
+
+
+def run_probe(name, torch, m, launches=4):
+    g = torch.Generator(device="cuda").manual_seed(0)
+    done = []
+    if name.startswith("fft"):
+        n = int(name[3:])
+        batch = (1 << 24) // n
+        f = m.MetalFFT(n)
+        x = torch.randn((batch, n), dtype=torch.complex64, device="cuda", generator=g)
+        y = torch.empty_like(x)
+        for _ in range(launches):
+            f.execute(x, y, batch)
+        done.append(f.kernel_name)
+    elif name.startswith("mfma"):
+        io = 1 if name.endswith("c32h") else 0
+        p = m.PrecisionFFT(4096, m.PrecisionFFT.FP16_MFMA_SPLIT, io)
+        x = torch.randn((4096, 4096), dtype=torch.complex64, device="cuda", generator=g)
+        if io:
+            x = torch.view_as_real(x).reshape(4096, 8192).half().contiguous()
+        y = torch.empty_like(x)
+        for _ in range(launches):
+            p.execute(x, y, 4096)
+        done.append(p.kernel_name)
+    elif name == "sar_fp16":
+        rda = m.RangeDoppler(m.RdaParams.defaults(n_range=4096, n_azimuth=4096, io=1, rcmc=1))
+        raw = (torch.randn((4096, 8192), generator=g, device="cuda") * 0.1).half()
+        img = torch.empty_like(raw)
+        for _ in range(launches):
+            rda.process(raw, img)
+        done.append("k_rows / k_corner_turn / k_rcmc")
+    elif name == "convolve":
+        K, S, cnt = 1023, 65536, 512
+        h = torch.randn(K, generator=g, device="cuda") / np.sqrt(K)
+        cv = m.FusedConvolver(h.cpu().numpy(), 4096)
+        x = torch.randn((cnt, S), generator=g, device="cuda")
+        y = torch.empty((cnt, S + K - 1), device="cuda")
+        gy = torch.randn_like(y)
+        gx = torch.empty_like(x)
+        gh = torch.empty(K, device="cuda")
+        for _ in range(launches):
+            cv.apply_device(x, y, S, cnt)
+            cv.backward_device(x, gy, gx, gh, S, cnt)
+        done.append("k_conv_fwd / k_conv_bwd_x / k_conv_bwd_h / k_sum_rows64")
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": name, "workload": PROBES[name], "launches": launches, "kernels": done}))
 
 
 def main():
@@ -525,6 +624,10 @@ def main():
         sys.exit(2)
     import torch
     import metalfft_amd as m
+
+    if a.probe:
+        run_probe(a.probe, torch, m)
+        return
 
     n, batch = a.n, a.batch
     fft = m.MetalFFT(n)
@@ -574,12 +677,14 @@ def main():
     # the reported form first (each form has its own warm-up): the other
     # form's 200 launches do not run right before the timed region
     if a.launch == "graph":
-        ms_graph, wall_graph, graph = time_graph(step, a.steps, a.warmup, torch)
+        ms_graph, wall_graph, graph, replays, settled = time_graph(
+            step, a.steps, a.warmup, torch, reps=a.replays, settle_s=a.settle)
         del graph
         ms_loop, wall_loop = time_gpu(step, a.steps, a.warmup, torch)
     else:
         ms_loop, wall_loop = time_gpu(step, a.steps, a.warmup, torch)
-        ms_graph, wall_graph, graph = time_graph(step, a.steps, a.warmup, torch)
+        ms_graph, wall_graph, graph, replays, settled = time_graph(
+            step, a.steps, a.warmup, torch, reps=a.replays, settle_s=a.settle)
         del graph
     ms, wall = (ms_graph, wall_graph) if a.launch == "graph" else (ms_loop, wall_loop)
     copy = copy_ceiling(sets, a.steps, torch)
@@ -621,6 +726,7 @@ def main():
                 extras[key] = {"error": repr(e)}
             torch.cuda.synchronize()
 
+    traffic, traffic_note = load_traffic(f"traffic_stockham_n{n}_b{batch}.json", fft.kernel_name)
     line = {
         "metric": METRIC,
         "value": value,
@@ -641,15 +747,19 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "copy_gbs": copy["gbs"],
                      "frac_of_copy": achieved / copy["gbs"],
-                     "traffic": load_traffic(f"traffic_stockham_n{n}_b{batch}.json")},
+                     "traffic": traffic, "traffic_source": traffic_note},
         "cpu_baseline": cpu,
         "rel_l2_vs_fp64": err,
         "gflops_5nlog2n": 5 * n * np.log2(n) * value / 1e9,
         "device": m.device_name(),
         "wall_s_timed": wall,
-        "launch": {"timed": "hipGraph replay of the K launches" if a.launch == "graph"
-                   else "K launches from a Python loop",
-                   "stream_loop_ms_per_step": ms_loop, "graph_replay_ms_per_step": ms_graph},
+        "launch": {"timed": (f"hipGraph of the K launches: {settled} untimed replays until the "
+                             f"replay time stopped falling, then {a.replays} replays back to back, "
+                             "each between its own HIP events on the replay stream; the median")
+                   if a.launch == "graph" else "K launches from a Python loop",
+                   "graph_replay_ms_per_step": ms_graph, "replays_ms_per_step": replays,
+                   "replay_min_max_ms_per_step": [min(replays), max(replays)],
+                   "settle_replays": settled, "stream_loop_ms_per_step": ms_loop},
         "extras": extras,
     }
     print(json.dumps(line))
