@@ -25,11 +25,20 @@ enum { kConvPure = 1, kConvStorage = 2, kConvMixed = 3 };
 __device__ __forceinline__ h2 to_h2(float2 a) { return (h2){(_Float16)a.x, (_Float16)a.y}; }
 __device__ __forceinline__ float2 to_f2(h2 a) { return make_float2((float)a.x, (float)a.y); }
 __device__ __forceinline__ float2 round_h(float2 a) { return to_f2(to_h2(a)); }
+// Complex product as two packed FP16 operations: a.x * (b.x, b.y) +
+// a.y * (-b.y, b.x).  The broadcasts and the swapped, negated operand are
+// op_sel / neg modifiers of v_pk_mul_f16 and v_pk_fma_f16, so no scalar
+// f16 multiply or repacking is issued (written component-wise, the compiler
+// scalarised every product).
 __device__ __forceinline__ h2 hmul(h2 a, h2 b) {
-    return (h2){a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    const h2 ax = __builtin_shufflevector(a, a, 0, 0);
+    const h2 ay = __builtin_shufflevector(a, a, 1, 1);
+    const h2 bs = __builtin_shufflevector(-b, b, 1, 2);  // (-b.y, b.x)
+    return __builtin_elementwise_fma(ay, bs, ax * b);
 }
-__device__ __forceinline__ h2 hmul_mi(h2 a) { return (h2){a.y, -a.x}; }
-__device__ __forceinline__ h2 hconj(h2 a) { return (h2){a.x, -a.y}; }
+// multiply by -i: (a.y, -a.x)
+__device__ __forceinline__ h2 hmul_mi(h2 a) { return __builtin_shufflevector(a, -a, 1, 2); }
+__device__ __forceinline__ h2 hconj(h2 a) { return __builtin_shufflevector(a, -a, 0, 3); }
 
 __device__ __forceinline__ void hdft4(h2& x0, h2& x1, h2& x2, h2& x3) {
     const h2 t0 = x0 + x2, t1 = x1 + x3, t2 = x0 - x2, t3 = hmul_mi(x1 - x3);
@@ -51,8 +60,11 @@ __device__ __forceinline__ void hdft16(h2* x) {
     }
     const _Float16 c = (_Float16)MIFFT_C16, sn = (_Float16)MIFFT_S16, h = (_Float16)MIFFT_SQRT1_2;
     const h2 w1 = (h2){c, -sn}, w3 = (h2){sn, -c}, w9 = (h2){-c, sn};
-    auto w2 = [&](h2 v) { return (h2){h * (v.x + v.y), h * (v.y - v.x)}; };
-    auto w6 = [&](h2 v) { return (h2){h * (v.y - v.x), -h * (v.x + v.y)}; };
+    const h2 hh = (h2){h, h};
+    // W16^2 v = h (v.x + v.y, v.y - v.x) = h (v + (v.y, -v.x));
+    // W16^6 v = h (v.y - v.x, -v.x - v.y) = h ((v.y, -v.x) - v)
+    auto w2 = [&](h2 v) { return (v + hmul_mi(v)) * hh; };
+    auto w6 = [&](h2 v) { return (hmul_mi(v) - v) * hh; };
     a[1][1] = hmul(a[1][1], w1);
     a[1][2] = w2(a[1][2]);
     a[1][3] = hmul(a[1][3], w3);
