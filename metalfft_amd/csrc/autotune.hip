@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "fourstep.hpp"
+#include "mfma_fft.hpp"
 #include "runtime.hpp"
 
 namespace mifft {
@@ -255,6 +256,22 @@ const StockhamVariant* choose_variant(int n, hipStream_t stream, const float2* t
     return q < 0 ? default_variant(n) : vs[q];
 }
 
+int choose_mfma(int precision, int io, hipStream_t stream, const float2* tw) {
+    static const char* prec_names[5] = {"fp32", "fp16", "bf16", "fp16split", "bf16split"};
+    if (mode() == "0" || precision < 1 || precision > 4) return kMfmaStaged;
+    std::vector<Candidate> cand;
+    for (int form : {kMfmaStaged, kMfmaOneShot})
+        cand.push_back({mfma_form_name(precision, io, form),
+                        [=](const void* in, void* out, int64_t batch) {
+                            return launch_fft_mfma(4096, precision, io, form, in, out, tw, batch, false, 1.0f,
+                                                   stream);
+                        },
+                        form == kMfmaOneShot});
+    const std::string key = std::string("mfma ") + prec_names[precision] + (io ? " c32h" : " c64") + " 4096";
+    const int q = tuned_choice(key, 4096, io ? 4 : 8, cand, stream, "mfma");
+    return q < 0 ? kMfmaStaged : q;
+}
+
 void choose_long(int n, hipStream_t stream, const float2* tw, const float2* fs_tw,
                  const StockhamVariant** variant, const FourStepVariant** fourstep) {
     *variant = nullptr;
@@ -327,6 +344,13 @@ int mifft_kernel_variants(int64_t n, const char** names, int cap) {
 
 mifft_status mifft_plan_select_kernel(mifft_plan* plan, const char* name) {
     if (!plan || !name) return mifft::fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
+    if (plan->precision != MIFFT_PREC_FP32) {
+        for (int form : {mifft::kMfmaStaged, mifft::kMfmaOneShot})
+            if (std::strcmp(mifft::mfma_form_name(plan->precision, plan->io, form), name) == 0) {
+                plan->mfma_form = form;
+                return MIFFT_OK;
+            }
+    }
     if (plan->precision == MIFFT_PREC_FP32) {
         if (plan->fs_twiddles) {
             int c = 0;

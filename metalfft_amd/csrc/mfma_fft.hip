@@ -142,15 +142,19 @@ __device__ __forceinline__ f4 gemm_half(typename MT<BF>::v8 a_hi, typename MT<BF
     return MT<BF>::mma(a_hi, b_hi, acc);
 }
 
-// Register budget for three resident workgroups per CU (3 waves per SIMD).
-template <bool BF, bool SPLIT, bool INV, class IO>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+// STAGE: persistent workgroups, the next transform staged HBM -> LDS by DMA
+// during the current one (register budget for three resident workgroups
+// per CU); otherwise one transform per workgroup loaded straight into
+// registers, no stage buffer (35 KiB of LDS: four workgroups per CU with
+// complex64 data, where the staged form fits two).
+template <bool BF, bool SPLIT, bool INV, class IO, bool STAGE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGE || !(SPLIT && !BF) ? 3 : 4, 8)))
 k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __restrict__ tw,
                int64_t batch, float out_scale) {
     constexpr int N = 4096;
     // stage: the next transform as it sits in HBM (DMA target, natural order);
     // lds: FP32 exchange buffer of the passes.
-    __shared__ typename IO::T stage[N];
+    __shared__ typename IO::T stage[STAGE ? N : 1];
     __shared__ float2 lds[kMfmaLds];
     __shared__ float red[4];
     const int lane = threadIdx.x & 63;
@@ -190,7 +194,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
     constexpr int kElemBytes = sizeof(typename IO::T);
     constexpr int kCount = N * kElemBytes / 8;  // transform size in float2 units (DMA granule)
     int64_t tr = blockIdx.x;
-    if (tr < batch)
+    if (STAGE && tr < batch)
         dma_transform(reinterpret_cast<float2*>(stage),
                       reinterpret_cast<const float2*>(in + tr * N), kCount, threadIdx.x, 256);
     for (; tr < batch; tr += gridDim.x) {  // uniform per workgroup
@@ -205,15 +209,20 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         const int c = lane & 15;
         const int g = lane >> 4;
         typename IO::T* dst = out + tr * N;
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA (and earlier stores) done
-        ds_barrier();                   // every wave's DMA landed; previous transform done
+        if constexpr (STAGE) {
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA (and earlier stores) done
+            ds_barrier();                   // every wave's DMA landed; previous transform done
+        }
+        // the transform as it sits in LDS (staged) or in HBM: 16 lanes of a
+        // row read 16 consecutive elements
+        const typename IO::T* src = STAGE ? static_cast<const typename IO::T*>(stage) : in + tr * N;
         float2 v[4][4];  // [tile][i]: x_{4g+i} of butterfly 16*(4w+tile) + c
         float m = 0.f;
 #pragma unroll
         for (int t4 = 0; t4 < 4; ++t4)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float2 x = IO::ld(stage + 16 * (4 * w + t4) + c + 256 * (4 * g + i));
+                float2 x = IO::ld(src + 16 * (4 * w + t4) + c + 256 * (4 * g + i));
                 if constexpr (INV) x.y = -x.y;
                 v[t4][i] = x;
                 m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
@@ -224,7 +233,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         if (lane == 0) red[w] = m;
         ds_barrier();  // stage fully read, red[] written
         m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        if (tr + gridDim.x < batch)
+        if (STAGE && tr + gridDim.x < batch)
             dma_transform(reinterpret_cast<float2*>(stage),
                           reinterpret_cast<const float2*>(in + (tr + gridDim.x) * N), kCount,
                           tid, 256);
@@ -318,53 +327,72 @@ int device_cus() {
     return cus;
 }
 
-template <bool BF, bool SPLIT, class IO>
+template <bool BF, bool SPLIT, class IO, bool STAGE>
 hipError_t launch_mfma(const void* in, void* out, const float2* tw, int64_t batch, bool inverse,
                        float out_scale, hipStream_t st) {
     if (batch <= 0) return hipSuccess;
-    // resident workgroups only (each loops over transforms, prefetching)
-    static int per_cu = [] {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &nb, reinterpret_cast<const void*>(&k_fft4096_mfma<BF, SPLIT, false, IO>), 256, 0) !=
-                hipSuccess || nb <= 0)
-            nb = 1;
-        return nb;
-    }();
-    const int64_t cap = (int64_t)device_cus() * per_cu;
-    const int64_t blocks = batch < cap ? batch : cap;
+    int64_t blocks = batch;  // one-shot: one transform per workgroup
+    if constexpr (STAGE) {
+        // resident workgroups only (each loops over transforms, prefetching)
+        static int per_cu = [] {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &nb, reinterpret_cast<const void*>(&k_fft4096_mfma<BF, SPLIT, false, IO, true>), 256,
+                    0) != hipSuccess || nb <= 0)
+                nb = 1;
+            return nb;
+        }();
+        const int64_t cap = (int64_t)device_cus() * per_cu;
+        if (blocks > cap) blocks = cap;
+    }
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     auto i = static_cast<const typename IO::T*>(in);
     auto o = static_cast<typename IO::T*>(out);
     if (inverse)
-        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, true, IO>), dim3((unsigned)blocks), dim3(256), 0,
-                           st, i, o, tw, batch, out_scale);
+        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, true, IO, STAGE>), dim3((unsigned)blocks), dim3(256),
+                           0, st, i, o, tw, batch, out_scale);
     else
-        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, false, IO>), dim3((unsigned)blocks), dim3(256), 0,
-                           st, i, o, tw, batch, out_scale);
+        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, false, IO, STAGE>), dim3((unsigned)blocks), dim3(256),
+                           0, st, i, o, tw, batch, out_scale);
     return hipGetLastError();
+}
+
+template <bool BF, bool SPLIT>
+hipError_t launch_mfma_io(int io, bool stage, const void* in, void* out, const float2* tw, int64_t batch,
+                          bool inverse, float out_scale, hipStream_t st) {
+    if (io == 0)
+        return stage ? launch_mfma<BF, SPLIT, IoC64, true>(in, out, tw, batch, inverse, out_scale, st)
+                     : launch_mfma<BF, SPLIT, IoC64, false>(in, out, tw, batch, inverse, out_scale, st);
+    return stage ? launch_mfma<BF, SPLIT, IoC32h, true>(in, out, tw, batch, inverse, out_scale, st)
+                 : launch_mfma<BF, SPLIT, IoC32h, false>(in, out, tw, batch, inverse, out_scale, st);
 }
 
 }  // namespace
 
-hipError_t launch_fft_mfma(int n, int precision, int io, const void* in, void* out, const float2* tw,
-                           int64_t batch, bool inverse, float out_scale, hipStream_t st) {
-    if (n != 4096) return hipErrorInvalidValue;
-    const bool c64 = io == 0;
+hipError_t launch_fft_mfma(int n, int precision, int io, int form, const void* in, void* out,
+                           const float2* tw, int64_t batch, bool inverse, float out_scale, hipStream_t st) {
+    if (n != 4096 || (io != 0 && io != 1)) return hipErrorInvalidValue;
+    const bool stage = form == kMfmaStaged;
     switch (precision) {
-        case 1:
-            return c64 ? launch_mfma<false, false, IoC64>(in, out, tw, batch, inverse, out_scale, st)
-                       : launch_mfma<false, false, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
-        case 2:
-            return c64 ? launch_mfma<true, false, IoC64>(in, out, tw, batch, inverse, out_scale, st)
-                       : launch_mfma<true, false, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
-        case 3:
-            return c64 ? launch_mfma<false, true, IoC64>(in, out, tw, batch, inverse, out_scale, st)
-                       : launch_mfma<false, true, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
-        case 4:
-            return c64 ? launch_mfma<true, true, IoC64>(in, out, tw, batch, inverse, out_scale, st)
-                       : launch_mfma<true, true, IoC32h>(in, out, tw, batch, inverse, out_scale, st);
+        case 1: return launch_mfma_io<false, false>(io, stage, in, out, tw, batch, inverse, out_scale, st);
+        case 2: return launch_mfma_io<true, false>(io, stage, in, out, tw, batch, inverse, out_scale, st);
+        case 3: return launch_mfma_io<false, true>(io, stage, in, out, tw, batch, inverse, out_scale, st);
+        case 4: return launch_mfma_io<true, true>(io, stage, in, out, tw, batch, inverse, out_scale, st);
     }
     return hipErrorInvalidValue;
+}
+
+const char* mfma_form_name(int precision, int io, int form) {
+    static const char* names[2][2][5] = {
+        {{"", "mfma_f16_4096_c64", "mfma_bf16_4096_c64", "mfma_f16split_4096_c64", "mfma_bf16split_4096_c64"},
+         {"", "mfma_f16_4096_c32h", "mfma_bf16_4096_c32h", "mfma_f16split_4096_c32h",
+          "mfma_bf16split_4096_c32h"}},
+        {{"", "mfma_f16_4096_c64_oneshot", "mfma_bf16_4096_c64_oneshot", "mfma_f16split_4096_c64_oneshot",
+          "mfma_bf16split_4096_c64_oneshot"},
+         {"", "mfma_f16_4096_c32h_oneshot", "mfma_bf16_4096_c32h_oneshot", "mfma_f16split_4096_c32h_oneshot",
+          "mfma_bf16split_4096_c32h_oneshot"}}};
+    if (precision < 1 || precision > 4 || io < 0 || io > 1 || form < 0 || form > 1) return "";
+    return names[form][io][precision];
 }
 
 }  // namespace mifft

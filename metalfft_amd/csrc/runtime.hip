@@ -140,7 +140,7 @@ mifft_status validate_exec(const mifft_plan* plan, const void* in, const void* o
 mifft_status run(const mifft_plan* plan, const float* in, float* out, int64_t batch, int dir,
                  hipStream_t stream) {
     if (plan->precision != MIFFT_PREC_FP32)
-        return check_hip(launch_fft_mfma((int)plan->n, plan->precision, plan->io, in, out,
+        return check_hip(launch_fft_mfma((int)plan->n, plan->precision, plan->io, plan->mfma_form, in, out,
                                          plan->twiddles, batch, dir == MIFFT_INVERSE, 1.0f, stream),
                          "mfma fft");
     if (plan->fourstep) {
@@ -334,9 +334,10 @@ mifft_status mifft_plan_create_ex(int64_t n, mifft_precision precision, mifft_io
         return MIFFT_OK;
     }
     // the Stockham variant serves FP32 plans (and their real-input entry
-    // points); MFMA plans run their own kernel, so only FP32 plans are timed
+    // points); MFMA plans time their own kernel forms instead
     p->variant = precision == MIFFT_PREC_FP32 ? choose_variant((int)n, p->stream, p->twiddles)
                                               : find_stockham((int)n, 16);
+    if (precision != MIFFT_PREC_FP32) p->mfma_form = choose_mfma(precision, io, p->stream, p->twiddles);
     if (!p->variant) {
         mifft_plan_destroy(p);
         return fail(MIFFT_ERR_KERNEL_NOT_FOUND, "HIP kernel not found for size " + std::to_string(n));
@@ -361,13 +362,8 @@ mifft_status mifft_plan_destroy(mifft_plan* plan) {
 int64_t mifft_plan_size(const mifft_plan* plan) { return plan ? plan->n : -1; }
 
 const char* mifft_plan_kernel_name(const mifft_plan* plan) {
-    static const char* mfma_names[2][5] = {
-        {"", "mfma_f16_4096_c64", "mfma_bf16_4096_c64", "mfma_f16split_4096_c64",
-         "mfma_bf16split_4096_c64"},
-        {"", "mfma_f16_4096_c32h", "mfma_bf16_4096_c32h", "mfma_f16split_4096_c32h",
-         "mfma_bf16split_4096_c32h"}};
     if (!plan) return "";
-    if (plan->precision != MIFFT_PREC_FP32) return mfma_names[plan->io][plan->precision];
+    if (plan->precision != MIFFT_PREC_FP32) return mfma_form_name(plan->precision, plan->io, plan->mfma_form);
     if (plan->fourstep) return plan->fourstep->name;
     return plan->variant ? plan->variant->name : "";
 }

@@ -64,6 +64,10 @@ int tuned_choice(const std::string& key, int n, size_t elem_bytes, const std::ve
 // False under MIFFT_AUTOTUNE=0 (built-in defaults, nothing timed).
 bool tuning_enabled();
 
+// MFMA plans (n = 4096): the faster kernel form (kMfmaStaged or
+// kMfmaOneShot) for the precision and device element type.
+int choose_mfma(int precision, int io, hipStream_t stream, const float2* tw);
+
 // Lengths above 16384: the fastest of the single-workgroup variants of n
 // (tw: their table, or null) and the four-step variants (fs_tw, or null);
 // exactly one of *variant / *fourstep is set.
@@ -78,6 +82,7 @@ struct mifft_plan {
     int io = 0;         // mifft_io
     const mifft::StockhamVariant* variant = nullptr;   // n <= 16384
     const mifft::FourStepVariant* fourstep = nullptr;  // n > 16384
+    int mfma_form = 0;                                 // MFMA plans: kMfmaStaged / kMfmaOneShot
     float2* twiddles = nullptr;     // single-workgroup kernels: W_N^k, k < N, then the
                                     // pass rows (twiddle_table_len); null if none exists
     float2* fs_twiddles = nullptr;  // four-step kernels (fourstep_table_len), in the same

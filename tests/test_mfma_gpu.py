@@ -112,3 +112,55 @@ def test_mfma_subnormal_row(gpu, oracle, prec):
     assert oracle.rel_l2(y[[0, 2, 3]], ref[[0, 2, 3]]) < bound
     # the subnormal row: FP16/BF16 keep ~2^-125 * 2^-24 absolute at worst
     assert np.max(np.abs(y[1] - ref[1])) < 1e-38
+
+
+FORMS = ("", "_oneshot")  # persistent DMA-staged, one transform per workgroup
+
+
+@pytest.mark.parametrize("prec,name,bound", PRECISIONS)
+@pytest.mark.parametrize("io", [0, 1])
+def test_mfma_every_form(gpu, oracle, tones, prec, name, bound, io):
+    """Both kernel forms of every precision and element type (the autotuner
+    picks one per plan; mifft_plan_select_kernel pins either) against FP64,
+    and the two forms agree to their rounding."""
+    import torch
+    from metalfft_amd import _lib
+    L = _lib.load()
+    plan = make_plan(gpu, prec, io)
+    chosen = L.mifft_plan_kernel_name(plan).decode()
+    base = chosen.replace("_oneshot", "")
+    assert base.startswith("mfma_")
+    if io == 0:
+        x = torch.from_numpy(tones).cuda()
+    else:
+        x = torch.from_numpy(tones.view(np.float32)).cuda().to(torch.float16)
+    ref = oracle.f64_fft(tones)
+    outs = []
+    for form in FORMS:
+        _lib.check(L.mifft_plan_select_kernel(plan, (base + form).encode()))
+        y = torch.empty_like(x)
+        run(gpu, plan, x, y, tones.shape[0])
+        yc = y.cpu().numpy() if io == 0 else y.float().cpu().numpy().view(np.complex64)
+        assert oracle.rel_l2(yc, ref) < max(bound, 1e-3 if io else 0), (name, io, form)
+        outs.append(yc)
+    assert oracle.rel_l2(outs[1], outs[0]) < max(bound, 1e-3 if io else 0)
+
+
+@pytest.mark.parametrize("prec,name,bound", PRECISIONS)
+def test_mfma_full_batch_4096(gpu, oracle, prec, name, bound):
+    """The metric's shape, N = 4096 x batch 4096 of the 20-tone workload, in
+    the tuned form: FP64 on 32 sampled transforms."""
+    import torch
+    from conftest import tones_c64
+    plan = make_plan(gpu, prec, 0)
+    rows = np.arange(0, 4096, 128)
+    x = torch.empty((4096, 4096), dtype=torch.complex64, device="cuda")
+    sample = tones_c64(len(rows), 4096, 77)
+    x.copy_(torch.repeat_interleave(torch.from_numpy(sample).cuda(), 4096 // len(rows), dim=0))
+    y = torch.empty_like(x)
+    run(gpu, plan, x, y, 4096)
+    got = y[torch.from_numpy(rows).cuda()].cpu().numpy()
+    src = x[torch.from_numpy(rows).cuda()].cpu().numpy()
+    err = oracle.rel_l2(got, oracle.f64_fft(src))
+    print(f"{name} batch 4096 rel_l2={err:.3e}")
+    assert err < bound
