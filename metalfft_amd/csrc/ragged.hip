@@ -1,20 +1,24 @@
 // Ragged batches: thousands of complex transforms of different power-of-two
-// lengths (64 ... 16384) in one persistent launch.
+// lengths (64 ... 16384) in one launch.
 //
 // No reference counterpart: MetalFFT.forward(batch:) rejects uneven lengths
 // (Sources/MetalFFT/FFT/MetalFFT.swift:108-114).
 //
-// Lengths of 4096 and longer run one transform per workgroup, one launch per
-// length (k_ragged_one, and k_ragged_half for 8192 and 16384: the uniform
-// kernels' bodies at a per-transform offset).  Shorter lengths are cut into tiles of 4096 elements (256 threads,
-// 32 KiB of LDS): 2 transforms of 2048, ..., 64 of 64 (the last tile of a
-// length may hold fewer); length N uses N/16 threads per transform
-// (radix-16 passes, 16 values per thread), so every tile keeps all its
-// threads and LDS busy whatever its length.  Tiles are sorted by length,
-// largest first, and one persistent launch (as many workgroups as are
-// resident) strides over them.  The same kernels launched once per length
-// are the one-launch-per-size baseline.  Both forms can be captured in a
-// hipGraph and replayed.
+// The transforms are sorted by length, longest first, and cut into tickets:
+// the transforms of one length that one 512-thread workgroup pass holds (one
+// of 16384, two of 8192, 8192 or 4096 elements of a shorter length; the tile
+// shape per length is measured at plan creation).  Launch structures over
+// the same ticket list, all capturable in a hipGraph:
+//   default    one launch, workgroup b runs ticket b (k_ragged_tickets);
+//   per size   the same kernel once per length;
+//   work queue one persistent launch drawing tickets from a device counter
+//              that resets itself (k_ragged_queue);
+//   split      one launch per length of 4096 and longer (one transform per
+//              workgroup, k_ragged_one / k_ragged_half) and one persistent
+//              launch over 4096-element tiles of the shorter lengths
+//              (k_ragged).
+// The three ticket forms run the same code per ticket and give the same
+// bits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
