@@ -121,6 +121,21 @@ static hipError_t launch_half(const float2* in, float2* out, const float2* tw, i
     return hipGetLastError();
 }
 
+// Half-LDS exchange with 16 values per thread (k_stockham_half16).
+template <int N, int WPE>
+static hipError_t launch_half16(const float2* in, float2* out, const float2* tw, int64_t batch,
+                                bool inverse, hipStream_t stream) {
+    if (batch <= 0) return hipSuccess;
+    if (batch > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (inverse)
+        hipLaunchKernelGGL((k_stockham_half16<N, true, true, WPE>), dim3((unsigned)batch),
+                           dim3(N / 16), 0, stream, in, out, tw, batch);
+    else
+        hipLaunchKernelGGL((k_stockham_half16<N, false, true, WPE>), dim3((unsigned)batch),
+                           dim3(N / 16), 0, stream, in, out, tw, batch);
+    return hipGetLastError();
+}
+
 #define MIFFT_V(N, E) \
     {N, E, tpw_for<N, E>(), &launch_stockham<N, E, LdsPad, false, false>, "stockham_r" #E "_" #N}
 #define MIFFT_VS(N, E)                                                                          \
@@ -147,6 +162,9 @@ static hipError_t launch_half(const float2* in, float2* out, const float2* tw, i
 // half-LDS exchange, E = 32, at least WPE waves per SIMD
 #define MIFFT_VH(N, WPE)                                                                        \
     {N, 32, 1, &launch_half<N, WPE>, "stockham_r16_" #N "_half" #WPE}
+// half-LDS exchange, E = 16, at least WPE waves per SIMD
+#define MIFFT_VH16(N, WPE)                                                                      \
+    {N, 16, 1, &launch_half16<N, WPE>, "stockham_r16_" #N "_half16w" #WPE}
 #define MIFFT_VD(N, E) \
     {N, E, tpw_for<N, E>(), &launch_dma<N, E>, "stockham_r" #E "_" #N "_dma"}
 
@@ -178,6 +196,7 @@ static const StockhamVariant kVariants[] = {
     // exchanged in halves (one transform per CU); the four-step kernels are
     // the other candidates for this length
     MIFFT_VH(32768, 4),
+    MIFFT_VH16(8192, 8), MIFFT_VH16(16384, 8),
 };
 #undef MIFFT_V
 #undef MIFFT_VS
@@ -187,6 +206,7 @@ static const StockhamVariant kVariants[] = {
 #undef MIFFT_VPT
 #undef MIFFT_VP3
 #undef MIFFT_VH
+#undef MIFFT_VH16
 
 const StockhamVariant* stockham_variants(int* count) {
     *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));

@@ -635,7 +635,7 @@ __device__ __forceinline__ void stockham_half(float2 (&v)[E], float2* buf, int t
     constexpr int B = E / R;
     constexpr int T = S::T;
     constexpr bool LAST = (P == S::passes - 1);
-    static_assert(E == 32 && B % 2 == 0, "two butterflies per thread and pass at least");
+    static_assert((E == 32 || E == 16) && (B % 2 == 0 || B == 1), "butterflies per thread and pass");
 
 #pragma unroll
     for (int i = 0; i < B; ++i) {
@@ -660,12 +660,20 @@ __device__ __forceinline__ void stockham_half(float2 (&v)[E], float2* buf, int t
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (P > 0 || h > 0) lds_barrier();  // the other half's reads are done
+            if constexpr (B >= 2) {
 #pragma unroll
-            for (int i = h * B / 2; i < (h + 1) * B / 2; ++i) {
-                const int b = t + i * T;
-                const int pos = b & (SD - 1);
-                const int base = (b / SD) * (R * SD) + pos - h * (N / 2);
-                lds_write_butterfly<R, SD, B, L>(buf, base, &v[i]);
+                for (int i = h * B / 2; i < (h + 1) * B / 2; ++i) {
+                    const int b = t + i * T;
+                    const int pos = b & (SD - 1);
+                    const int base = (b / SD) * (R * SD) + pos - h * (N / 2);
+                    lds_write_butterfly<R, SD, B, L>(buf, base, &v[i]);
+                }
+            } else if ((t < T / 2) == (h == 0)) {
+                // one butterfly per thread (b = t): half 0 iff t < N/(2R) = T/2,
+                // whole waves on either side
+                const int pos = t & (SD - 1);
+                const int base = (t / SD) * (R * SD) + pos - h * (N / 2);
+                lds_write_butterfly<R, SD, B, L>(buf, base, &v[0]);
             }
             lds_barrier();
             lds_read_column<E / 2, T, L>(&u[h * E / 2], buf, t);  // positions h*N/2 + t + T*m
@@ -689,6 +697,31 @@ __device__ __forceinline__ void half_transform(const float2* src, float2* dst,
     half_bases<N, E, 1>(w, tw, t);  // pass 1's rows, in flight during pass 0
     stockham_half<N, E, LdsSwz, 0>(v, lds, t, tw, w);
     if (active) store_column<N, E, INV, NT>(v, dst, t);
+}
+
+// The same exchange with 16 values per thread (N/16 threads, one radix-16
+// butterfly per thread and pass): twice the waves per transform, for the
+// same N/2 entries of LDS.
+template <int N, bool INV, bool NT>
+__device__ __forceinline__ void half16_transform(const float2* src, float2* dst,
+                                                 const float2* __restrict__ tw, float2* lds, int t,
+                                                 bool active) {
+    constexpr int E = 16;
+    float2 v[E];
+    load_column<N, E, INV, NT>(v, src, t, active);
+    float2 w[E / 16 * 4];
+    half_bases<N, E, 1>(w, tw, t);
+    stockham_half<N, E, LdsSwz, 0>(v, lds, t, tw, w);
+    if (active) store_column<N, E, INV, NT>(v, dst, t);
+}
+
+template <int N, bool INV, bool NT, int WPE>
+__global__ void __launch_bounds__(N / 16)
+__attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_stockham_half16(const float2* in, float2* out, const float2* __restrict__ tw, int64_t batch) {
+    __shared__ float2 lds[N / 2];
+    const int64_t tr = blockIdx.x;
+    half16_transform<N, INV, NT>(in + tr * N, out + tr * N, tw, lds, threadIdx.x, tr < batch);
 }
 
 // One transform per workgroup of N/32 threads, LDS N/2 entries.  WPE: waves

@@ -102,6 +102,38 @@ def test_split_exchange_halves(n):
             assert (t + t_count * m) // (n // 2) == m // 16
 
 
+@pytest.mark.parametrize("n", [8192, 16384])
+def test_split_exchange_halves_e16(n):
+    """E = 16 (k_stockham_half16): passes with several butterflies per thread
+    split them as for E = 32; radix-16 passes (one butterfly per thread)
+    split by thread, half 0 written by threads t < T/2 (whole waves), each
+    half exactly once; half h is read back as v[m], m in [8h, 8h + 8); the
+    XOR write identity holds for the shifted bases."""
+    e, t_count = 16, n // 16
+    radix, stride = schedule(n, e)
+    for r, s in zip(radix[:-1], stride[:-1]):
+        bpt = e // r
+        hits = [0] * n
+        for t in range(t_count):
+            for i in range(bpt):
+                b = t + i * t_count
+                base = (b // s) * (r * s) + (b % s)
+                h = i // (bpt // 2) if bpt >= 2 else (0 if t < t_count // 2 else 1)
+                if bpt == 1:
+                    assert (t < t_count // 2) == (b < n // (2 * r))
+                    assert (t_count // 2) % 64 == 0  # whole waves on either side
+                sb = base - h * (n // 2)
+                for j in range(r):
+                    p = base + j * s
+                    assert p // (n // 2) == h
+                    assert swz(sb + j * s) == swz(sb) ^ swz(j * s)
+                    hits[p] += 1
+        assert hits == [1] * n
+    for t in range(t_count):
+        for m in range(e):
+            assert (t + t_count * m) // (n // 2) == m // 8
+
+
 def test_mfma_pass_addresses():
     """The DFT-as-GEMM kernel's closed-form LDS addresses
     (metalfft_amd/csrc/mfma_fft.hip) equal midx() of the Stockham positions
