@@ -147,16 +147,23 @@ k_fourstep_cols(const float2* in, float2* y, const float2* __restrict__ tw, int6
     const int local = tid / T, t = tid % T;
     float2 v[16];
     column_fft<N2>(v, lds + local * Tl::LEN, t, tw + sub_off(N2));
-    const int n1 = c0 + local;
+    // n1 is the same for a whole wave when a column spans whole waves
+    const int n1 = T >= 64 ? __builtin_amdgcn_readfirstlane(c0 + local) : c0 + local;
     const float2* lo = tw + kSubTotal;
     const float2* hi = lo + (1 << S);
     float2* dst = y + tr * N + (int64_t)n1 * N2;
+    // W_N^{n1 k2}, k2 = t + T m, as W_N^{n1 t} W_N^{n1 T m}: one gathered
+    // factor pair per thread, the other per m the same for the column (one
+    // cache line per wave load instead of up to 64); exponents < N, no
+    // reduction mod N
+    const int ea = n1 * t;
+    const float2 wa = cmul(hi[ea >> S], lo[ea & ((1 << S) - 1)]);
+    gstore<true>(dst + t, cmul(v[0], wa));
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int k2 = t + T * m;
-        const int e = n1 * k2;  // < N: no reduction mod N
-        const float2 w = cmul(hi[e >> S], lo[e & ((1 << S) - 1)]);
-        gstore<true>(dst + k2, cmul(v[m], w));
+    for (int m = 1; m < 16; ++m) {
+        const int eb = n1 * (T * m);
+        const float2 w = cmul(wa, cmul(hi[eb >> S], lo[eb & ((1 << S) - 1)]));
+        gstore<true>(dst + t + T * m, cmul(v[m], w));
     }
 }
 
