@@ -88,30 +88,44 @@ struct AzimuthFilter {
     }
 };
 
-// The same filter read from a table H[row][k] built once per processor
-// (k_az_table: the phase above in FP64 with an FP64 sincospi, rounded once to
-// FP32): a read in place of an FP64 phase and an FP32 sincos per element.
+// The same filter from two tables built once per processor: the phase is
+// linear in the row, so with row = 64 rh + rl
+//   H[row][k] = E1[rh][k] E2[rl][k],
+//   E1[rh][k] = exp(i s 2 pi (A[k] + (64 rh - Nr/2) B D[k])),
+//   E2[rl][k] = exp(i s 2 pi rl B D[k]),
+// each entry's phase in FP64 with an FP64 sincospi, rounded once to FP32
+// (k_az_factors), zero where the filter is.  Two reads that stay in L2 (64
+// rows of E2 and the current rows of E1: 4 MiB at 4096 Doppler bins) and one
+// complex product per element, instead of an FP64 phase and an FP32 sincos,
+// or of a read of an Nr x Na table from HBM (128 MiB at 4096 x 4096, as much
+// as the frame itself).
+constexpr int kAzRowBits = 6;
 struct AzimuthTable {
-    const float2* H;
+    const float2* E1;
+    const float2* E2;
     int64_t n;  // Doppler bins per row
-    __device__ __forceinline__ float2 at(int64_t row, int k) const { return H[row * n + k]; }
+    __device__ __forceinline__ float2 at(int64_t row, int k) const {
+        return cmul(E1[(row >> kAzRowBits) * n + k], E2[(row & ((1 << kAzRowBits) - 1)) * n + k]);
+    }
 };
 
-__global__ void k_az_table(const double* __restrict__ A, const double* __restrict__ D, double B, int64_t half_nr,
-                           double sign, int64_t rows, int64_t n, float2* __restrict__ H) {
+// rows_hi rows of E1, then 64 rows of E2.
+__global__ void k_az_factors(const double* __restrict__ A, const double* __restrict__ D, double B, int64_t half_nr,
+                             double sign, int64_t rows_hi, int64_t n, float2* __restrict__ E) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows * n) return;
-    const int64_t row = i / n, k = i % n;
+    if (i >= (rows_hi + (1 << kAzRowBits)) * n) return;
+    const int64_t r = i / n, k = i % n;
     const double d = D[k];
     float2 h = make_float2(0.f, 0.f);
     if (d >= 0.0) {
-        double ph = A[k] + (double)(row - half_nr) * B * d;
+        double ph = r < rows_hi ? A[k] + (double)((r << kAzRowBits) - half_nr) * B * d
+                                : (double)(r - rows_hi) * B * d;
         ph -= rint(ph);
         double sn, cs;
         sincospi(2.0 * ph, &sn, &cs);
         h = make_float2((float)cs, (float)(sign * sn));
     }
-    H[i] = h;
+    E[i] = h;
 }
 
 template <int N>
@@ -461,9 +475,12 @@ template <class F>
 mifft_status rda_azimuth_with(const mifft_rda* r, const F& f, const void* in, void* out, hipStream_t st);
 
 mifft_status rda_azimuth(const mifft_rda* r, const void* in, void* out, hipStream_t st) {
-    if (r->az_table.ptr)
-        return rda_azimuth_with(r, AzimuthTable{static_cast<const float2*>(r->az_table.ptr), r->p.n_azimuth}, in,
-                                out, st);
+    if (r->az_table.ptr) {
+        const float2* e1 = static_cast<const float2*>(r->az_table.ptr);
+        const int64_t rows_hi = (r->p.n_range + (1 << kAzRowBits) - 1) >> kAzRowBits;
+        return rda_azimuth_with(r, AzimuthTable{e1, e1 + rows_hi * r->p.n_azimuth, r->p.n_azimuth}, in, out,
+                                st);
+    }
     AzimuthFilter f{static_cast<const double*>(r->az_a.ptr), static_cast<const double*>(r->az_d.ptr),
                     r->az_b, r->p.n_range / 2,
                     r->p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0f : 1.0f};
@@ -578,15 +595,17 @@ mifft_status rda_setup(mifft_rda* r) {
         (s = check_hip(hipMemcpy(r->az_d.ptr, D.data(), ab, hipMemcpyHostToDevice), "hipMemcpy")) !=
             MIFFT_OK)
         return s;
-    // the filter as a table when it is at most 512 MiB (4096 x 4096: 128 MiB)
-    const int64_t cells = p.n_range * p.n_azimuth;
+    // the filter's two factor tables, (Nr / 64 + 64) x Na entries (4 MiB at
+    // 4096 x 4096), when they hold at most 2^26 entries
+    const int64_t rows_hi = (p.n_range + (1 << kAzRowBits) - 1) >> kAzRowBits;
+    const int64_t cells = (rows_hi + (1 << kAzRowBits)) * p.n_azimuth;
     if (cells <= ((int64_t)1 << 26)) {
         if ((s = r->az_table.reserve(sizeof(float2) * (size_t)cells)) != MIFFT_OK) return s;
-        hipLaunchKernelGGL(k_az_table, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, r->stream,
+        hipLaunchKernelGGL(k_az_factors, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, r->stream,
                            static_cast<const double*>(r->az_a.ptr), static_cast<const double*>(r->az_d.ptr), r->az_b,
-                           p.n_range / 2, p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0 : 1.0, p.n_range,
+                           p.n_range / 2, p.filter_convention == MIFFT_RDA_REFERENCE_FORMULA ? -1.0 : 1.0, rows_hi,
                            p.n_azimuth, static_cast<float2*>(r->az_table.ptr));
-        if ((s = check_hip(hipGetLastError(), "azimuth filter table")) != MIFFT_OK ||
+        if ((s = check_hip(hipGetLastError(), "azimuth filter tables")) != MIFFT_OK ||
             (s = check_hip(hipStreamSynchronize(r->stream), "hipStreamSynchronize")) != MIFFT_OK)
             return s;
     }

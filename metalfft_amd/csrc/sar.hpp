@@ -8,7 +8,7 @@ struct mifft_rda {
     mifft_plan* az_plan = nullptr;     // length n_azimuth
     mifft::DeviceBuffer range_filter;  // conj(FFT(chirp)), complex64
     mifft::DeviceBuffer az_a, az_d;    // FP64 azimuth phase model per Doppler bin
-    mifft::DeviceBuffer az_table;      // H[range bin][Doppler bin], FP32 (frames up to 2^26 cells)
+    mifft::DeviceBuffer az_table;      // azimuth filter factor tables E1 (Nr/64 rows), E2 (64 rows), FP32
     double az_b = 0.0;                 // cycles per range bin per unit D
     double range_scale = 1.0, az_scale = 1.0;
     int64_t chirp_samples = 0;
