@@ -67,13 +67,11 @@ struct SarIo<MIFFT_IO_C32H> {
 };
 
 struct RangeFilter {
-    static constexpr bool kPrefetch = true;  // one read per element: loaded with the row
     const float2* H;  // conj(FFT(chirp reference)), n entries
     __device__ __forceinline__ float2 at(int64_t, int k) const { return H[k]; }
 };
 
 struct AzimuthFilter {
-    static constexpr bool kPrefetch = false;
     const double* A;  // frac(2 R0c D(k) / lambda), cycles
     const double* D;  // sqrt(1 - ratio^2), or -1 where the filter is zero
     double B;         // 2 * range pixel / lambda, cycles per bin per unit D
@@ -103,7 +101,6 @@ struct AzimuthFilter {
 // as the frame itself).
 constexpr int kAzRowBits = 6;
 struct AzimuthTable {
-    static constexpr bool kPrefetch = false;  // two reads per element: at use
     const float2* E1;
     const float2* E2;
     int64_t n;  // Doppler bins per row
@@ -203,18 +200,11 @@ k_rows(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2
     float2 v[E];
     RegTwiddles<N, E> rt;  // table entries in registers for both transforms
     rt.load(tw, t);
-    constexpr bool PF = MODE == kRowsCompress && F::kPrefetch;
-    float2 hf[PF ? E : 1];
     {
         const typename SarIo<IO>::T* src = in + (active ? row : 0) * N;
 #pragma unroll
         for (int m = 0; m < E; ++m)
             v[m] = active ? SarIo<IO>::ld(src + t + T * m) : make_float2(0.f, 0.f);
-        if constexpr (PF) {
-            // the filter, in flight during the forward transform
-#pragma unroll
-            for (int m = 0; m < E; ++m) hf[m] = filt.at(row, t + T * m);
-        }
         if constexpr (MODE != kRowsFilterInverse) rt.template fft<false>(v, buf, t, tw);
     }
     if constexpr (MODE == kRowsForward) {
@@ -228,8 +218,7 @@ k_rows(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2
         {
             const int tf = fresh(t);  // filter index arithmetic not shared with the loads
 #pragma unroll
-            for (int m = 0; m < E; ++m)
-                v[m] = conjf2(cmul(v[m], PF ? hf[m] : filt.at(row, tf + T * m)));
+            for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], filt.at(row, tf + T * m)));
         }
         rt.template fft<MODE == kRowsCompress>(v, buf, t, tw);  // SYNC0: LDS used by the forward
         if (!active) return;
