@@ -256,6 +256,20 @@ struct NoDraw {
     __device__ __forceinline__ void publish() {}
 };
 
+// With T >= 64 threads per transform every wave works on one transform:
+// its offset moves to scalar registers, so the loads and stores address
+// from a scalar base (no 64-bit pointer per lane).
+template <bool UNIFORM>
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+    if constexpr (UNIFORM) {
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+        return (int64_t)(((uint64_t)hi << 32) | lo);
+    } else {
+        return v;
+    }
+}
+
 // One ticket: `count` transforms of length 2^L at sorted offsets
 // offs[first ...]; a slot past `count` transforms the ticket's last
 // transform again and stores nothing.
@@ -272,7 +286,7 @@ __device__ __forceinline__ void ticket_body(const float2* in, float2* out, const
         const int h = L == 14 ? 0 : tid >> 8;
         const int t = L == 14 ? tid : tid & 255;
         const bool active = h < count;
-        const int64_t off = offs[first + (active ? h : count - 1)];
+        const int64_t off = wave_uniform<true>(offs[first + (active ? h : count - 1)]);
         float2 v[E];
         load_column<N, E, INV, true>(v, in + off, t, true);
         td.publish();
@@ -284,7 +298,7 @@ __device__ __forceinline__ void ticket_body(const float2* in, float2* out, const
         constexpr int E = SHAPE ? 8 : 16;
         constexpr int T = N / E;
         const int local = tid / T, t = tid % T;
-        const int64_t off = offs[first + (local < count ? local : count - 1)];
+        const int64_t off = wave_uniform<T >= 64>(offs[first + (local < count ? local : count - 1)]);
         float2 v[E];
         load_column<N, E, INV, true>(v, in + off, t, true);
         td.publish();
