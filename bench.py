@@ -3,8 +3,9 @@
 One step = one batched forward transform of 4096 length-4096 complex64
 signals (128 MiB in, 128 MiB out) already resident in HBM.  Input sets
 rotate over >= 1 GiB so the 256 MiB Infinity Cache cannot hold a step's
-working set.  Prints one JSON line (driver contract); extra keys carry the
-roofline, the CPU baseline and the reduced-precision forms.
+working set.  Prints one JSON line (metric, value, timing, configuration);
+extra keys carry the roofline, the CPU baseline and the reduced-precision
+forms.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20]
 """
