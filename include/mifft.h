@@ -291,7 +291,9 @@ typedef struct mifft_ragged_plan mifft_ragged_plan;
                                      free up */
 #define MIFFT_RAGGED_PER_SIZE 1   /* one launch per distinct length (baseline) */
 #define MIFFT_RAGGED_GRAPH 2      /* capture the launch(es) in a hipGraph once per
-                                     (in, out, direction, mode) and replay it */
+                                     (in, out, direction, mode) and replay it; a plan
+                                     keeps up to four such graphs, replacing the least
+                                     recently used */
 #define MIFFT_RAGGED_QUEUE 4      /* one persistent launch: the resident workgroups take
                                      tickets from a device counter (the plan's counter
                                      makes calls on one plan exclusive) */

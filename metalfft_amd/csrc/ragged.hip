@@ -587,11 +587,14 @@ int choose_shape(int l, const float2* tw_all, hipStream_t st) {
     return q < 0 ? fallback : q;
 }
 
-void drop_graph(mifft_ragged_plan* rp) {
-    if (rp->graph_exec) (void)hipGraphExecDestroy(rp->graph_exec);
-    if (rp->graph) (void)hipGraphDestroy(rp->graph);
-    rp->graph_exec = nullptr;
-    rp->graph = nullptr;
+void drop_slot(mifft_ragged_plan::GraphSlot& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = mifft_ragged_plan::GraphSlot{};
+}
+
+void drop_graphs(mifft_ragged_plan* rp) {
+    for (auto& g : rp->graphs) drop_slot(g);
 }
 
 }  // namespace
@@ -709,7 +712,7 @@ mifft_status mifft_ragged_plan_create(const int64_t* lengths, const int64_t* off
 mifft_status mifft_ragged_plan_destroy(mifft_ragged_plan* rp) {
     if (!rp) return MIFFT_OK;
     if (rp->stream) (void)hipStreamSynchronize(rp->stream);
-    drop_graph(rp);
+    drop_graphs(rp);
     rp->d_tiles.release();
     rp->d_tickets.release();
     rp->d_queue.release();
@@ -756,9 +759,14 @@ mifft_status mifft_ragged_execute_device(mifft_ragged_plan* rp, const float* in,
     if (!(mode & MIFFT_RAGGED_GRAPH)) return enqueue(rp, i2, o2, inverse, mode, st);
 
     // Graph form: captured once per (buffers, direction, mode), replayed after.
-    if (!rp->graph_exec || rp->g_in != in || rp->g_out != out || rp->g_dir != dir ||
-        rp->g_mode != mode) {
-        drop_graph(rp);
+    mifft_ragged_plan::GraphSlot* slot = nullptr;
+    for (auto& g : rp->graphs)
+        if (g.exec && g.in == in && g.out == out && g.dir == dir && g.mode == mode) slot = &g;
+    if (!slot) {
+        slot = &rp->graphs[0];  // an empty slot, else the least recently used
+        for (auto& g : rp->graphs)
+            if (!g.exec || (slot->exec && g.used < slot->used)) slot = &g;
+        drop_slot(*slot);
         hipStream_t cap = rp->stream;
         mifft_status s =
             check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -770,18 +778,19 @@ mifft_status mifft_ragged_execute_device(mifft_ragged_plan* rp, const float* in,
             if (g) (void)hipGraphDestroy(g);
             return se != MIFFT_OK ? se : s;
         }
-        rp->graph = g;
-        if ((s = check_hip(hipGraphInstantiate(&rp->graph_exec, g, nullptr, nullptr, 0),
-                           "hipGraphInstantiate")) != MIFFT_OK) {
-            drop_graph(rp);
+        slot->graph = g;
+        if ((s = check_hip(hipGraphInstantiate(&slot->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate")) !=
+            MIFFT_OK) {
+            drop_slot(*slot);
             return s;
         }
-        rp->g_in = in;
-        rp->g_out = out;
-        rp->g_dir = dir;
-        rp->g_mode = mode;
+        slot->in = in;
+        slot->out = out;
+        slot->dir = dir;
+        slot->mode = mode;
     }
-    return check_hip(hipGraphLaunch(rp->graph_exec, st), "hipGraphLaunch");
+    slot->used = ++rp->graph_clock;
+    return check_hip(hipGraphLaunch(slot->exec, st), "hipGraphLaunch");
 }
 
 }  // extern "C"

@@ -1,5 +1,6 @@
 // Ragged-batch plan shared by ragged.hip and the C-ABI.
 #pragma once
+#include <cstdint>
 #include <vector>
 
 #include "runtime.hpp"
@@ -40,9 +41,18 @@ struct mifft_ragged_plan {
     int shape[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     mifft::DeviceBuffer d_tickets, d_queue;
     hipStream_t stream = nullptr;  // capture stream
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    const void* g_in = nullptr;
-    const void* g_out = nullptr;
-    int g_dir = 0, g_mode = -1;
+    // graph form: one captured graph per (in, out, direction, mode), the
+    // least recently used one replaced when all slots are taken (a forward
+    // and an inverse on two buffer pairs alternate without re-capturing)
+    struct GraphSlot {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const void* in = nullptr;
+        const void* out = nullptr;
+        int dir = 0, mode = -1;
+        uint64_t used = 0;
+    };
+    static constexpr int kGraphSlots = 4;
+    GraphSlot graphs[kGraphSlots];
+    uint64_t graph_clock = 0;
 };

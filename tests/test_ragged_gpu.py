@@ -134,6 +134,40 @@ def test_graph_replays_match_eager(gpu, oracle):
         _ = (xd * 2).sum().item()  # unrelated eager work between replays
 
 
+def test_graph_cache_alternating_keys(gpu, oracle):
+    """Forward x -> y and inverse y -> z alternate in graph mode (two cached
+    graphs), then six buffer/direction keys cycle through the four slots
+    (eviction and re-capture): every replay equals the eager result."""
+    import torch
+    lengths = ragged_lengths(400, 9)
+    x = ragged_data(lengths, 9)
+    rp = gpu.RaggedFFT(lengths)
+    xd = torch.from_numpy(x).cuda()
+    ref_y = torch.empty_like(xd)
+    rp.execute(xd, ref_y)
+    ref_z = torch.empty_like(xd)
+    rp.execute(ref_y, ref_z, direction=gpu.INVERSE)
+    torch.cuda.synchronize()
+    y = torch.empty_like(xd)
+    z = torch.empty_like(xd)
+    for r in range(3):
+        y.fill_(float("nan"))
+        z.fill_(float("nan"))
+        rp.execute(xd, y, mode=gpu.RAGGED_GRAPH)
+        rp.execute(y, z, direction=gpu.INVERSE, mode=gpu.RAGGED_GRAPH)
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref_y) and torch.equal(z, ref_z), r
+    outs = [torch.empty_like(xd) for _ in range(6)]
+    for r in range(2):
+        for k, o in enumerate(outs):
+            o.fill_(float("nan"))
+            rp.execute(xd, o, mode=gpu.RAGGED_GRAPH | (gpu.RAGGED_QUEUE if k % 2 else 0))
+        torch.cuda.synchronize()
+        for k, o in enumerate(outs):
+            assert torch.equal(o, ref_y), (r, k)
+    assert oracle.rel_l2(ref_z.cpu().numpy(), x) < 1e-6
+
+
 @pytest.mark.parametrize("lengths", [[64, 4096, 128, 1024] * 40, [8192, 64] * 31, [16384] * 5,
                                      [16384, 8192] * 7 + [8192], [2048] * 3 + [8192], [64] * 129,
                                      [4096] * 3, [256, 512, 1024, 2048] * 50])
