@@ -8,7 +8,7 @@
 //
 // Choices are kept in memory for the life of the process.  A disk cache is
 // opt-in: only when $MIFFT_AUTOTUNE_CACHE names a file is it read and
-// written (text, first line "mifft-autotune 7 <arch>", then one
+// written (text, first line "mifft-autotune 8 <arch>", then one
 // "<kind> <precision> <io> <n> <variant-name> <ns-per-transform>" line per
 // entry, e.g. "stockham fp32 c64 4096 ..."; a
 // header mismatch discards it); without it the library writes nothing.
@@ -37,7 +37,7 @@
 namespace mifft {
 
 namespace {
-const char* kHeader = "mifft-autotune 7 gfx950";
+const char* kHeader = "mifft-autotune 8 gfx950";
 
 std::mutex g_tune_mutex;
 bool g_loaded = false;
@@ -260,13 +260,13 @@ int choose_mfma(int precision, int io, hipStream_t stream, const float2* tw) {
     static const char* prec_names[5] = {"fp32", "fp16", "bf16", "fp16split", "bf16split"};
     if (mode() == "0" || precision < 1 || precision > 4) return kMfmaStaged;
     std::vector<Candidate> cand;
-    for (int form : {kMfmaStaged, kMfmaOneShot})
+    for (int form = 0; form < kMfmaForms; ++form)
         cand.push_back({mfma_form_name(precision, io, form),
                         [=](const void* in, void* out, int64_t batch) {
                             return launch_fft_mfma(4096, precision, io, form, in, out, tw, batch, false, 1.0f,
                                                    stream);
                         },
-                        form == kMfmaOneShot});
+                        form == kMfmaOneShot || form == kMfmaOneShot64});
     const std::string key = std::string("mfma ") + prec_names[precision] + (io ? " c32h" : " c64") + " 4096";
     const int q = tuned_choice(key, 4096, io ? 4 : 8, cand, stream, "mfma");
     return q < 0 ? kMfmaStaged : q;
@@ -345,7 +345,7 @@ int mifft_kernel_variants(int64_t n, const char** names, int cap) {
 mifft_status mifft_plan_select_kernel(mifft_plan* plan, const char* name) {
     if (!plan || !name) return mifft::fail(MIFFT_ERR_INVALID_ARGUMENT, "null argument");
     if (plan->precision != MIFFT_PREC_FP32) {
-        for (int form : {mifft::kMfmaStaged, mifft::kMfmaOneShot})
+        for (int form = 0; form < mifft::kMfmaForms; ++form)
             if (std::strcmp(mifft::mfma_form_name(plan->precision, plan->io, form), name) == 0) {
                 plan->mfma_form = form;
                 return MIFFT_OK;

@@ -27,6 +27,16 @@
 //    range through the three passes (growth <= 16 per pass); the output is
 //    multiplied back by 2^e, which is exact.
 //
+//  * Radix-64 form (R64): 4096 = 64 x 64, two self-sorting passes of 64
+//    radix-64 butterflies, each radix-64 butterfly B = 16 w + c done as four
+//    DFT-16s on the matrix cores (tile t4 = j4 takes X[j4 + 4 j16], column
+//    c, so one lane holds the four sub-transforms' outputs at the same k16)
+//    followed, in FP32 registers, by the twiddle W_64^{j4 k16} and a DFT-4
+//    across the tiles:  Y[k16 + 16 k4] = sum_j4 W_4^{j4 k4} W_64^{j4 k16}
+//    Z_j4[k16].  One LDS exchange and two operand splits per transform
+//    instead of two and three, the same MFMA tiles per pass, two passes
+//    instead of three.
+//
 // Lane layout of v_mfma_f32_16x16x32 (checked on the device): A lane l holds
 // row l%16, K = 8*(l/16) .. +7; B lane l holds column l%16, the same K range;
 // D lane l holds column l%16, rows 4*(l/16) .. +3.  So lane (c = l%16,
@@ -103,6 +113,16 @@ struct IoC32h {
 __device__ __forceinline__ int midx(int i) { return i + (i >> 4) + ((i >> 10) << 4); }
 constexpr int kMfmaLds = 4096 + 256 + 64;
 
+// Radix-64 form's LDS index: bits 6-9 of a spread onto bits 0, 1, 3, 4 and
+// bit 10 onto bit 4 (XOR, a bijection on [0, 4096)).  Pass 0 writes
+// 64 (16 w + c) + 4 g + i + 16 k4 and pass 1 reads 16 w + c + 64 (t4 + 16 g
+// + 4 i); in both, the 32 lanes of a half-wave land on 32 distinct 8-byte
+// slots (checked on the CPU for every lane and element).
+__device__ __forceinline__ int r64idx(int a) {
+    const int m = ((a >> 6) & 3) | (((a >> 8) & 1) << 3) | ((((a >> 9) ^ (a >> 10)) & 1) << 4);
+    return a ^ m;
+}
+
 template <bool BF, bool SPLIT>
 __device__ __forceinline__ void split8(const float (&x)[8], typename MT<BF>::v8& hi,
                                        typename MT<BF>::v8& lo) {
@@ -147,15 +167,16 @@ __device__ __forceinline__ f4 gemm_half(typename MT<BF>::v8 a_hi, typename MT<BF
 // per CU); otherwise one transform per workgroup loaded straight into
 // registers, no stage buffer (35 KiB of LDS: four workgroups per CU with
 // complex64 data, where the staged form fits two).
-template <bool BF, bool SPLIT, bool INV, class IO, bool STAGE>
+template <bool BF, bool SPLIT, bool INV, class IO, bool STAGE, bool R64>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STAGE || !(SPLIT && !BF) ? 3 : 4, 8)))
 k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __restrict__ tw,
                int64_t batch, float out_scale) {
     constexpr int N = 4096;
+    constexpr int PASSES = R64 ? 2 : 3;
     // stage: the next transform as it sits in HBM (DMA target, natural order);
     // lds: FP32 exchange buffer of the passes.
     __shared__ typename IO::T stage[STAGE ? N : 1];
-    __shared__ float2 lds[kMfmaLds];
+    __shared__ float2 lds[R64 ? N : kMfmaLds];
     __shared__ float red[4];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -180,15 +201,29 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         split8<BF, SPLIT>(fi, aim_hi, aim_lo);
     }
     // stage twiddles W_4096^{j*pos*N/(16 s)}: pass 1 (s = 16) has pos = c for
-    // every tile, pass 2 (s = 256) pos = 16*(4w + tile) + c
+    // every tile, pass 2 (s = 256) pos = 16*(4w + tile) + c.
+    // R64: pass 1's W_4096^{J B} (J = t4 + 16 g + 4 i, B = 16 w + c) as
+    // W^{B (16 g + 4 i)} (tw1[i]) times W^{B t4} (tw2[0][t4]); the in-butterfly
+    // W_64^{t4 k16} (k16 = 4 g + i) as tw2[1][i] = W_64^{k16}, squared and
+    // multiplied at use.
     float2 tw1[4], tw2[4][4];
+    if constexpr (R64) {
+        const int B = 16 * w + c;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) tw1[i] = tw[(4 * g + i) * c * 16];
+        for (int i = 0; i < 4; ++i) {
+            tw1[i] = tw[B * (16 * g + 4 * i)];
+            tw2[0][i] = tw[B * i];
+            tw2[1][i] = tw[64 * (4 * g + i)];
+        }
+    } else {
 #pragma unroll
-    for (int t4 = 0; t4 < 4; ++t4) {
-        const int pos = 16 * (4 * w + t4) + c;
+        for (int i = 0; i < 4; ++i) tw1[i] = tw[(4 * g + i) * c * 16];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) tw2[t4][i] = tw[(4 * g + i) * pos];
+        for (int t4 = 0; t4 < 4; ++t4) {
+            const int pos = 16 * (4 * w + t4) + c;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) tw2[t4][i] = tw[(4 * g + i) * pos];
+        }
     }
 
     constexpr int kElemBytes = sizeof(typename IO::T);
@@ -222,7 +257,9 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         for (int t4 = 0; t4 < 4; ++t4)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float2 x = IO::ld(src + 16 * (4 * w + t4) + c + 256 * (4 * g + i));
+                // R64: X_J of butterfly 16 w + c, J = t4 + 16 g + 4 i
+                float2 x = IO::ld(src + (R64 ? 16 * w + c + 64 * t4 + 1024 * g + 256 * i
+                                             : 16 * (4 * w + t4) + c + 256 * (4 * g + i)));
                 if constexpr (INV) x.y = -x.y;
                 v[t4][i] = x;
                 m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
@@ -254,7 +291,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         const int wr0 = 1104 * w + 17 * c + 4 * g;  // pass 0 writes
         const int wr1 = 1104 * w + c + 68 * g;      // pass 1 writes
 #pragma unroll
-        for (int p = 0; p < 3; ++p) {
+        for (int p = 0; p < PASSES; ++p) {
             const int s = p == 0 ? 1 : (p == 1 ? 16 : 256);
             if (p > 0) {
 #pragma unroll
@@ -262,8 +299,13 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
                     const int b = 16 * (4 * w + t4) + c;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const int a = CLOSED ? rd0 + 17 * t4 + 272 * i : midx(b + 256 * (4 * g + i));
-                        v[t4][i] = cmul(lds[a], p == 1 ? tw1[i] : tw2[t4][i]);
+                        if constexpr (R64) {
+                            const float2 wj = t4 == 0 ? tw1[i] : cmul(tw1[i], tw2[0][t4]);
+                            v[t4][i] = cmul(lds[r64idx(16 * w + c + 64 * (t4 + 16 * g + 4 * i))], wj);
+                        } else {
+                            const int a = CLOSED ? rd0 + 17 * t4 + 272 * i : midx(b + 256 * (4 * g + i));
+                            v[t4][i] = cmul(lds[a], p == 1 ? tw1[i] : tw2[t4][i]);
+                        }
                     }
                 }
             }
@@ -283,7 +325,26 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
 #pragma unroll
                 for (int i = 0; i < 4; ++i) y[t4][i] = make_float2(yr[i], yi[i]);
             }
-            if (p < 2) {
+            if constexpr (R64) {
+                // y[t4][i] = Z_t4[k16], k16 = 4 g + i: times W_64^{t4 k16}, then
+                // the DFT-4 over t4 gives Y[k16 + 16 k4] in y[k4][i]
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float2 w1 = tw2[1][i];
+                    const float2 w2 = cmul(w1, w1);
+                    const float2 a0 = y[0][i], a1 = cmul(y[1][i], w1), a2 = cmul(y[2][i], w2),
+                                 a3 = cmul(y[3][i], cmul(w1, w2));
+                    const float2 s02 = make_float2(a0.x + a2.x, a0.y + a2.y);
+                    const float2 d02 = make_float2(a0.x - a2.x, a0.y - a2.y);
+                    const float2 s13 = make_float2(a1.x + a3.x, a1.y + a3.y);
+                    const float2 d13 = make_float2(a1.x - a3.x, a1.y - a3.y);
+                    y[0][i] = make_float2(s02.x + s13.x, s02.y + s13.y);
+                    y[2][i] = make_float2(s02.x - s13.x, s02.y - s13.y);
+                    y[1][i] = make_float2(d02.x + d13.y, d02.y - d13.x);  // d02 - i d13
+                    y[3][i] = make_float2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
+                }
+            }
+            if (p < PASSES - 1) {
                 if (p > 0) ds_barrier();  // all reads of lds for this pass are done
 #pragma unroll
                 for (int t4 = 0; t4 < 4; ++t4) {
@@ -291,14 +352,19 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
                     const int base = (b / s) * (16 * s) + (b & (s - 1));
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const int a = !CLOSED ? midx(base + (4 * g + i) * s)
-                                      : p == 0 ? wr0 + 272 * t4 + i : wr1 + 272 * t4 + 17 * i;
+                        int a;
+                        if constexpr (R64)  // Y[4 g + i + 16 t4] of butterfly 16 w + c, stride 1
+                            a = r64idx(64 * (16 * w + c) + 4 * g + i + 16 * t4);
+                        else
+                            a = !CLOSED ? midx(base + (4 * g + i) * s)
+                                : p == 0 ? wr0 + 272 * t4 + i : wr1 + 272 * t4 + 17 * i;
                         lds[a] = y[t4][i];
                     }
                 }
                 ds_barrier();
             } else {
-                // last pass: y_k of butterfly b goes to b + 256 k
+                // last pass: y_k of butterfly b goes to b + 256 k (R64: Y_K of
+                // butterfly 16 w + c, K = 4 g + i + 16 t4, to 16 w + c + 64 K)
                 float osc = ldexpf(out_scale, e);
                 if constexpr (INV) osc *= 1.0f / (float)N;
 #pragma unroll
@@ -308,7 +374,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
                     for (int i = 0; i < 4; ++i) {
                         float2 r = make_float2(y[t4][i].x * osc, y[t4][i].y * osc);
                         if constexpr (INV) r.y = -r.y;
-                        IO::st(dst + b + 256 * (4 * g + i), r);
+                        IO::st(dst + (R64 ? 16 * w + c + 64 * (4 * g + i + 16 * t4) : b + 256 * (4 * g + i)), r);
                     }
                 }
             }
@@ -327,7 +393,7 @@ int device_cus() {
     return cus;
 }
 
-template <bool BF, bool SPLIT, class IO, bool STAGE>
+template <bool BF, bool SPLIT, class IO, bool STAGE, bool R64>
 hipError_t launch_mfma(const void* in, void* out, const float2* tw, int64_t batch, bool inverse,
                        float out_scale, hipStream_t st) {
     if (batch <= 0) return hipSuccess;
@@ -337,7 +403,7 @@ hipError_t launch_mfma(const void* in, void* out, const float2* tw, int64_t batc
         static int per_cu = [] {
             int nb = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &nb, reinterpret_cast<const void*>(&k_fft4096_mfma<BF, SPLIT, false, IO, true>), 256,
+                    &nb, reinterpret_cast<const void*>(&k_fft4096_mfma<BF, SPLIT, false, IO, true, R64>), 256,
                     0) != hipSuccess || nb <= 0)
                 nb = 1;
             return nb;
@@ -349,49 +415,66 @@ hipError_t launch_mfma(const void* in, void* out, const float2* tw, int64_t batc
     auto i = static_cast<const typename IO::T*>(in);
     auto o = static_cast<typename IO::T*>(out);
     if (inverse)
-        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, true, IO, STAGE>), dim3((unsigned)blocks), dim3(256),
+        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, true, IO, STAGE, R64>), dim3((unsigned)blocks), dim3(256),
                            0, st, i, o, tw, batch, out_scale);
     else
-        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, false, IO, STAGE>), dim3((unsigned)blocks), dim3(256),
-                           0, st, i, o, tw, batch, out_scale);
+        hipLaunchKernelGGL((k_fft4096_mfma<BF, SPLIT, false, IO, STAGE, R64>), dim3((unsigned)blocks),
+                           dim3(256), 0, st, i, o, tw, batch, out_scale);
     return hipGetLastError();
 }
 
+template <bool BF, bool SPLIT, class IO>
+hipError_t launch_mfma_form(int form, const void* in, void* out, const float2* tw, int64_t batch, bool inverse,
+                            float out_scale, hipStream_t st) {
+    switch (form) {
+        case kMfmaStaged: return launch_mfma<BF, SPLIT, IO, true, false>(in, out, tw, batch, inverse, out_scale, st);
+        case kMfmaOneShot: return launch_mfma<BF, SPLIT, IO, false, false>(in, out, tw, batch, inverse, out_scale, st);
+        case kMfmaStaged64: return launch_mfma<BF, SPLIT, IO, true, true>(in, out, tw, batch, inverse, out_scale, st);
+        case kMfmaOneShot64:
+            return launch_mfma<BF, SPLIT, IO, false, true>(in, out, tw, batch, inverse, out_scale, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 template <bool BF, bool SPLIT>
-hipError_t launch_mfma_io(int io, bool stage, const void* in, void* out, const float2* tw, int64_t batch,
+hipError_t launch_mfma_io(int io, int form, const void* in, void* out, const float2* tw, int64_t batch,
                           bool inverse, float out_scale, hipStream_t st) {
-    if (io == 0)
-        return stage ? launch_mfma<BF, SPLIT, IoC64, true>(in, out, tw, batch, inverse, out_scale, st)
-                     : launch_mfma<BF, SPLIT, IoC64, false>(in, out, tw, batch, inverse, out_scale, st);
-    return stage ? launch_mfma<BF, SPLIT, IoC32h, true>(in, out, tw, batch, inverse, out_scale, st)
-                 : launch_mfma<BF, SPLIT, IoC32h, false>(in, out, tw, batch, inverse, out_scale, st);
+    if (io == 0) return launch_mfma_form<BF, SPLIT, IoC64>(form, in, out, tw, batch, inverse, out_scale, st);
+    return launch_mfma_form<BF, SPLIT, IoC32h>(form, in, out, tw, batch, inverse, out_scale, st);
 }
 
 }  // namespace
 
 hipError_t launch_fft_mfma(int n, int precision, int io, int form, const void* in, void* out,
                            const float2* tw, int64_t batch, bool inverse, float out_scale, hipStream_t st) {
-    if (n != 4096 || (io != 0 && io != 1)) return hipErrorInvalidValue;
-    const bool stage = form == kMfmaStaged;
+    if (n != 4096 || (io != 0 && io != 1) || form < 0 || form >= kMfmaForms) return hipErrorInvalidValue;
     switch (precision) {
-        case 1: return launch_mfma_io<false, false>(io, stage, in, out, tw, batch, inverse, out_scale, st);
-        case 2: return launch_mfma_io<true, false>(io, stage, in, out, tw, batch, inverse, out_scale, st);
-        case 3: return launch_mfma_io<false, true>(io, stage, in, out, tw, batch, inverse, out_scale, st);
-        case 4: return launch_mfma_io<true, true>(io, stage, in, out, tw, batch, inverse, out_scale, st);
+        case 1: return launch_mfma_io<false, false>(io, form, in, out, tw, batch, inverse, out_scale, st);
+        case 2: return launch_mfma_io<true, false>(io, form, in, out, tw, batch, inverse, out_scale, st);
+        case 3: return launch_mfma_io<false, true>(io, form, in, out, tw, batch, inverse, out_scale, st);
+        case 4: return launch_mfma_io<true, true>(io, form, in, out, tw, batch, inverse, out_scale, st);
     }
     return hipErrorInvalidValue;
 }
 
 const char* mfma_form_name(int precision, int io, int form) {
-    static const char* names[2][2][5] = {
+    static const char* names[kMfmaForms][2][5] = {
         {{"", "mfma_f16_4096_c64", "mfma_bf16_4096_c64", "mfma_f16split_4096_c64", "mfma_bf16split_4096_c64"},
          {"", "mfma_f16_4096_c32h", "mfma_bf16_4096_c32h", "mfma_f16split_4096_c32h",
           "mfma_bf16split_4096_c32h"}},
         {{"", "mfma_f16_4096_c64_oneshot", "mfma_bf16_4096_c64_oneshot", "mfma_f16split_4096_c64_oneshot",
           "mfma_bf16split_4096_c64_oneshot"},
          {"", "mfma_f16_4096_c32h_oneshot", "mfma_bf16_4096_c32h_oneshot", "mfma_f16split_4096_c32h_oneshot",
-          "mfma_bf16split_4096_c32h_oneshot"}}};
-    if (precision < 1 || precision > 4 || io < 0 || io > 1 || form < 0 || form > 1) return "";
+          "mfma_bf16split_4096_c32h_oneshot"}},
+        {{"", "mfma_f16_4096_c64_r64", "mfma_bf16_4096_c64_r64", "mfma_f16split_4096_c64_r64",
+          "mfma_bf16split_4096_c64_r64"},
+         {"", "mfma_f16_4096_c32h_r64", "mfma_bf16_4096_c32h_r64", "mfma_f16split_4096_c32h_r64",
+          "mfma_bf16split_4096_c32h_r64"}},
+        {{"", "mfma_f16_4096_c64_r64_oneshot", "mfma_bf16_4096_c64_r64_oneshot",
+          "mfma_f16split_4096_c64_r64_oneshot", "mfma_bf16split_4096_c64_r64_oneshot"},
+         {"", "mfma_f16_4096_c32h_r64_oneshot", "mfma_bf16_4096_c32h_r64_oneshot",
+          "mfma_f16split_4096_c32h_r64_oneshot", "mfma_bf16split_4096_c32h_r64_oneshot"}}};
+    if (precision < 1 || precision > 4 || io < 0 || io > 1 || form < 0 || form >= kMfmaForms) return "";
     return names[form][io][precision];
 }
 

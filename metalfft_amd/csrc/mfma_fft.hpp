@@ -5,8 +5,10 @@
 
 namespace mifft {
 // Kernel forms: persistent workgroups with the next transform staged in LDS
-// by DMA, or one transform per workgroup loaded straight into registers.
-enum { kMfmaStaged = 0, kMfmaOneShot = 1 };
+// by DMA, or one transform per workgroup loaded straight into registers;
+// three radix-16 passes, or two radix-64 passes (each four DFT-16s on the
+// matrix cores and a DFT-4 in registers).
+enum { kMfmaStaged = 0, kMfmaOneShot = 1, kMfmaStaged64 = 2, kMfmaOneShot64 = 3, kMfmaForms = 4 };
 // precision: 1 FP16, 2 BF16, 3 FP16 hi/lo split, 4 BF16 hi/lo split
 // (mifft_precision); io: 0 complex64, 1 complex FP16 (mifft_io).  n = 4096.
 // out = out_scale * FFT(in) (inverse: conj(FFT(conj in)) * out_scale / n).

@@ -154,6 +154,73 @@ def test_mfma_pass_addresses():
                             1104 * w + c + 68 * g + 272 * t4 + 17 * i
 
 
+def test_mfma_radix64_mapping():
+    """The radix-64 MFMA form (metalfft_amd/csrc/mfma_fft.hip, R64): pass 0
+    reads X_J of butterfly B = 16 w + c at B + 64 J, J = t4 + 16 g + 4 i
+    (tile t4 = j4, MFMA column c, K range j16 = 4 g + i), four DFT-16s give
+    Z_t4[k16] (k16 = 4 g + i in the lane's D rows), the twiddle W_64^{t4 k16}
+    and a DFT-4 over t4 give Y[k16 + 16 k4], written to 64 B + K; pass 1
+    reads the same positions with W_4096^{J B} and writes B + 64 K.  Run in
+    FP64 with exactly these indices it is the DFT; r64idx is a bijection and
+    every half-wave of either access lands on 32 distinct 8-byte slots."""
+    import numpy as np
+    n = 4096
+
+    def r64idx(a):
+        m = ((a >> 6) & 3) | (((a >> 8) & 1) << 3) | ((((a >> 9) ^ (a >> 10)) & 1) << 4)
+        return a ^ m
+
+    tw = np.exp(-2j * np.pi * np.arange(n) / n)
+    f16 = np.exp(-2j * np.pi * np.outer(np.arange(16), np.arange(16)) / 16)
+    rng = np.random.default_rng(64)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    lds = np.zeros(n, complex)
+    out = np.zeros(n, complex)
+    for p in range(2):
+        new = np.zeros(n, complex)
+        for w in range(4):
+            for c in range(16):
+                b = 16 * w + c
+                z = np.zeros((4, 16), complex)
+                for t4 in range(4):
+                    xs = np.zeros(16, complex)
+                    for g in range(4):
+                        for i in range(4):
+                            j = t4 + 16 * g + 4 * i
+                            if p == 0:
+                                xs[4 * g + i] = x[16 * w + c + 64 * t4 + 1024 * g + 256 * i]
+                            else:
+                                wj = tw[b * (16 * g + 4 * i)] * tw[b * t4]
+                                xs[4 * g + i] = lds[r64idx(16 * w + c + 64 * j)] * wj
+                    z[t4] = f16 @ xs
+                for k16 in range(16):
+                    w1 = tw[64 * k16]
+                    a0, a1, a2, a3 = z[0, k16], z[1, k16] * w1, z[2, k16] * w1 * w1, z[3, k16] * w1 ** 3
+                    y = [a0 + a1 + a2 + a3, (a0 - a2) - 1j * (a1 - a3), (a0 + a2) - (a1 + a3),
+                         (a0 - a2) + 1j * (a1 - a3)]
+                    for k4 in range(4):
+                        kk = k16 + 16 * k4
+                        if p == 0:
+                            new[r64idx(64 * b + kk)] = y[k4]
+                        else:
+                            out[b + 64 * kk] = y[k4]
+        lds = new
+    ref = np.fft.fft(x)
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-13
+    a = np.arange(n)
+    assert sorted(r64idx(a).tolist()) == list(range(n))
+    for w in range(4):
+        for h in range(2):
+            half = [(c, g) for c in range(16) for g in (2 * h, 2 * h + 1)]
+            for i in range(4):
+                for k4 in range(4):
+                    slots = {r64idx(64 * (16 * w + c) + 4 * g + i + 16 * k4) % 32 for c, g in half}
+                    assert len(slots) == 32
+                for t4 in range(4):
+                    slots = {r64idx(16 * w + c + 64 * (t4 + 16 * g + 4 * i)) % 32 for c, g in half}
+                    assert len(slots) == 32
+
+
 def test_padded_layout_identities():
     """LdsPad (one spare entry per 16): idx(t + T m) = idx(t) + (T + T/16) m
     for T a multiple of 16, and idx(base + j s) = idx(base) + j (s + s/16)

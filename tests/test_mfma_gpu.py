@@ -114,21 +114,23 @@ def test_mfma_subnormal_row(gpu, oracle, prec):
     assert np.max(np.abs(y[1] - ref[1])) < 1e-38
 
 
-FORMS = ("", "_oneshot")  # persistent DMA-staged, one transform per workgroup
+# persistent DMA-staged or one transform per workgroup, three radix-16 or
+# two radix-64 passes
+FORMS = ("", "_oneshot", "_r64", "_r64_oneshot")
 
 
 @pytest.mark.parametrize("prec,name,bound", PRECISIONS)
 @pytest.mark.parametrize("io", [0, 1])
 def test_mfma_every_form(gpu, oracle, tones, prec, name, bound, io):
-    """Both kernel forms of every precision and element type (the autotuner
-    picks one per plan; mifft_plan_select_kernel pins either) against FP64,
-    and the two forms agree to their rounding."""
+    """Every kernel form of every precision and element type (the autotuner
+    picks one per plan; mifft_plan_select_kernel pins any) against FP64,
+    and the forms agree with the first to their rounding."""
     import torch
     from metalfft_amd import _lib
     L = _lib.load()
     plan = make_plan(gpu, prec, io)
     chosen = L.mifft_plan_kernel_name(plan).decode()
-    base = chosen.replace("_oneshot", "")
+    base = chosen.replace("_oneshot", "").replace("_r64", "")
     assert base.startswith("mfma_")
     if io == 0:
         x = torch.from_numpy(tones).cuda()
@@ -143,7 +145,8 @@ def test_mfma_every_form(gpu, oracle, tones, prec, name, bound, io):
         yc = y.cpu().numpy() if io == 0 else y.float().cpu().numpy().view(np.complex64)
         assert oracle.rel_l2(yc, ref) < max(bound, 1e-3 if io else 0), (name, io, form)
         outs.append(yc)
-    assert oracle.rel_l2(outs[1], outs[0]) < max(bound, 1e-3 if io else 0)
+    for o in outs[1:]:
+        assert oracle.rel_l2(o, outs[0]) < max(bound, 1e-3 if io else 0)
 
 
 @pytest.mark.parametrize("prec,name,bound", PRECISIONS)
