@@ -175,8 +175,18 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
     constexpr int PASSES = R64 ? 2 : 3;
     // stage: the next transform as it sits in HBM (DMA target, natural order);
     // lds: FP32 exchange buffer of the passes.
+    // SHARED (staged radix-64 FP16 hi/lo form, complex64 data): the stage
+    // buffer is also the exchange buffer.  The next transform's DMA is issued
+    // once pass 1 has read the exchange (overlapping pass 1's MFMAs and
+    // stores instead of the whole transform), for 32 KiB of LDS instead of
+    // 64: three workgroups per CU instead of two (53 against 57.5 us per
+    // 4096 x 4096).  The other precisions need more than the 168 registers of
+    // three waves per SIMD there and spill, and keep two buffers.
+    constexpr bool SHARED = STAGE && R64 && SPLIT && !BF && sizeof(typename IO::T) == sizeof(float2);
     __shared__ typename IO::T stage[STAGE ? N : 1];
-    __shared__ float2 lds[R64 ? N : kMfmaLds];
+    __shared__ float2 lds_own[SHARED ? 1 : (R64 ? N : kMfmaLds)];
+    static_assert(!SHARED || sizeof(stage) >= N * sizeof(float2), "the shared buffer holds the exchange");
+    float2* const lds = SHARED ? reinterpret_cast<float2*>(stage) : lds_own;
     __shared__ float red[4];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -270,7 +280,7 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
         if (lane == 0) red[w] = m;
         ds_barrier();  // stage fully read, red[] written
         m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        if (STAGE && tr + gridDim.x < batch)
+        if (STAGE && !SHARED && tr + gridDim.x < batch)
             dma_transform(reinterpret_cast<float2*>(stage),
                           reinterpret_cast<const float2*>(in + (tr + gridDim.x) * N), kCount,
                           tid, 256);
@@ -307,6 +317,13 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
                             v[t4][i] = cmul(lds[a], p == 1 ? tw1[i] : tw2[t4][i]);
                         }
                     }
+                }
+                if constexpr (SHARED) {  // the exchange is read: the buffer takes the next transform
+                    ds_barrier();
+                    if (tr + gridDim.x < batch)
+                        dma_transform(reinterpret_cast<float2*>(stage),
+                                      reinterpret_cast<const float2*>(in + (tr + gridDim.x) * N), kCount, tid,
+                                      256);
                 }
             }
             float2 y[4][4];
