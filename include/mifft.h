@@ -137,7 +137,11 @@ MIFFT_API const char *mifft_plan_kernel_name(const mifft_plan *plan);
  * replaced by an autotuner over these; see mifft_plan_select_kernel. */
 MIFFT_API int mifft_kernel_variants(int64_t n, const char **names, int cap);
 /* Pins the plan to one named variant (overrides the autotuner's choice);
- * MIFFT_ERR_KERNEL_NOT_FOUND for an unknown name or another length. */
+ * MIFFT_ERR_KERNEL_NOT_FOUND for an unknown name or another length.  Plans
+ * of another precision (mifft_plan_create_ex) take the names of their four
+ * DFT-as-GEMM forms, "mfma_<f16|bf16|f16split|bf16split>_4096_<c64|c32h>"
+ * followed by nothing (DMA-staged, three radix-16 passes), "_oneshot",
+ * "_r64" (two radix-64 passes) or "_r64_oneshot". */
 MIFFT_API mifft_status mifft_plan_select_kernel(mifft_plan *plan, const char *name);
 
 /* Batched transform of `batch` contiguous length-n complex64 signals on the
