@@ -50,16 +50,15 @@ struct SarIo;
 template <>
 struct SarIo<MIFFT_IO_C64> {
     using T = float2;
+    static __device__ __forceinline__ float2 cvt(T v) { return v; }
     static __device__ __forceinline__ float2 ld(const T* p) { return *p; }
     static __device__ __forceinline__ void st(T* p, float2 v) { *p = v; }
 };
 template <>
 struct SarIo<MIFFT_IO_C32H> {
     using T = h2;
-    static __device__ __forceinline__ float2 ld(const T* p) {
-        const h2 v = *p;
-        return make_float2((float)v.x, (float)v.y);
-    }
+    static __device__ __forceinline__ float2 cvt(T v) { return make_float2((float)v.x, (float)v.y); }
+    static __device__ __forceinline__ float2 ld(const T* p) { return cvt(*p); }
     static __device__ __forceinline__ void st(T* p, float2 v) {
         h2 o = {(_Float16)v.x, (_Float16)v.y};
         *p = o;
@@ -126,6 +125,26 @@ __global__ void k_az_factors(const double* __restrict__ A, const double* __restr
         h = make_float2((float)cs, (float)(sign * sn));
     }
     E[i] = h;
+}
+
+int sar_cu_count() {
+    static int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256;
+        return n > 0 ? n : 256;
+    }();
+    return cus;
+}
+
+// Row kernel form at N = 4096: persistent with the next row prefetched
+// (default), or one workgroup per row (MIFFT_SAR_ROWS=oneshot).
+bool sar_rows_stream() {
+    static bool on = [] {
+        const char* e = std::getenv("MIFFT_SAR_ROWS");
+        return !(e && std::string(e) == "oneshot");
+    }();
+    return on;
 }
 
 template <int N>
@@ -227,6 +246,71 @@ k_rows(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2
         const int ts = fresh(t);
 #pragma unroll
         for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + ts + T * m, make_float2(v[m].x * s, -v[m].y * s));
+    }
+}
+
+// Persistent form of k_rows for one row per 256-thread workgroup (E = 16,
+// N = 4096): the grid is the resident workgroups, each loops over rows
+// blockIdx.x, + gridDim.x, ... and issues the loads of its next row before
+// the transforms of the current one, so a row's HBM read overlaps the
+// previous row's arithmetic instead of every resident workgroup loading at
+// once and then computing at once.  The prefetched row is held as loaded
+// (a complex-FP16 row is 16 VGPRs per thread).  The next row is always
+// loaded (the current one again past the end: in bounds, never used), so no
+// load sits under a branch.  Rows are read whole before they are written
+// and a workgroup's prefetch is of another row than any being written, so
+// in == out is allowed as in k_rows.
+template <int N, int IO, class F, int MODE>
+__global__ void __launch_bounds__(N / 16)
+k_rows_stream(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2* __restrict__ tw,
+              F filt, int64_t rows, float scale) {
+    constexpr int E = 16;
+    constexpr int T = N / E;
+    static_assert(tpw_for<N, E>() == 1, "one row per workgroup");
+    using Tp = typename SarIo<IO>::T;
+    __shared__ float2 lds[LdsPad::len(N)];
+    const int t = threadIdx.x;
+    const int64_t step = gridDim.x;  // the launch keeps gridDim.x <= rows
+    RegTwiddles<N, E> rt;
+    rt.load(tw, t);
+    Tp nxt[E];
+    {
+        const Tp* src = in + (int64_t)blockIdx.x * N;
+#pragma unroll
+        for (int m = 0; m < E; ++m) nxt[m] = src[t + T * m];
+    }
+    // the trip count depends on blockIdx only: barriers stay uniform
+    for (int64_t row = blockIdx.x; row < rows; row += step) {
+        const int tl = fresh(t);  // per-iteration index arithmetic (not hoisted)
+        float2 v[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = SarIo<IO>::cvt(nxt[m]);
+        {
+            const int64_t rn = row + step < rows ? row + step : row;
+            const Tp* src = in + rn * N;
+#pragma unroll
+            for (int m = 0; m < E; ++m) nxt[m] = src[tl + T * m];
+        }
+        // SYNC0 on every transform: the previous row's passes used the LDS
+        if constexpr (MODE != kRowsFilterInverse) rt.template fft<true>(v, lds, tl, tw);
+        Tp* dst = out + row * N;
+        if constexpr (MODE == kRowsForward) {
+            const int ts = fresh(t);
+#pragma unroll
+            for (int m = 0; m < E; ++m)
+                SarIo<IO>::st(dst + ts + T * m, make_float2(v[m].x * scale, v[m].y * scale));
+        } else {
+            {
+                const int tf = fresh(t);
+#pragma unroll
+                for (int m = 0; m < E; ++m) v[m] = conjf2(cmul(v[m], filt.at(row, tf + T * m)));
+            }
+            rt.template fft<true>(v, lds, fresh(t), tw);
+            const float s = scale / (float)N;
+            const int ts = fresh(t);
+#pragma unroll
+            for (int m = 0; m < E; ++m) SarIo<IO>::st(dst + ts + T * m, make_float2(v[m].x * s, -v[m].y * s));
+        }
     }
 }
 
@@ -388,6 +472,24 @@ hipError_t rows_n(const void* in, void* out, const float2* tw, F f, int64_t rows
     constexpr int TPW = tpw_for<N, E>();
     const int64_t blocks = (rows + TPW - 1) / TPW;
     if (blocks <= 0) return hipSuccess;
+    if constexpr (N == 4096) {
+        // persistent form when the rows outnumber the resident workgroups
+        static int per_cu = [] {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &nb, reinterpret_cast<const void*>(&k_rows_stream<N, IO, F, MODE>), N / 16, 0) !=
+                    hipSuccess || nb <= 0)
+                nb = 1;
+            return nb;
+        }();
+        const int64_t resident = (int64_t)per_cu * sar_cu_count();
+        if (sar_rows_stream() && rows > resident) {
+            hipLaunchKernelGGL((k_rows_stream<N, IO, F, MODE>), dim3((unsigned)resident), dim3(N / 16), 0, st,
+                               static_cast<const typename SarIo<IO>::T*>(in),
+                               static_cast<typename SarIo<IO>::T*>(out), tw, f, rows, scale);
+            return hipGetLastError();
+        }
+    }
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_rows<N, IO, F, MODE>), dim3((unsigned)blocks), dim3((N / E) * TPW), 0, st,
                        static_cast<const typename SarIo<IO>::T*>(in),
