@@ -111,8 +111,9 @@ MIFFT_API mifft_status mifft_device_name(char *buf, size_t len);
 /* ------------------------------------------------------ plan (MetalFFT) */
 typedef struct mifft_plan mifft_plan;
 
-/* MetalFFT.init(size:) (MetalFFT.swift:27-51): n in {64, ..., 16384} powers of
- * two, else MIFFT_ERR_UNSUPPORTED_SIZE (FFTDescriptor, MetalContext.swift:48-67).
+/* MetalFFT.init(size:) (MetalFFT.swift:27-51): n a power of two from 64 to
+ * 16384 as FFTDescriptor (MetalContext.swift:48-67), or 32768 ... 2^20 (the
+ * four-step lengths), else MIFFT_ERR_UNSUPPORTED_SIZE.
  * Creation allocates the plan's twiddle tables and stream and, for FP32
  * plans, the first creation of a length in a process times the compiled
  * kernel variants of that length (about 1 GiB of temporary device memory
