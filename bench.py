@@ -567,6 +567,7 @@ PROBES = {
     "mfma_fp16split_c64": "DFT-as-GEMM FP16 hi/lo, N=4096, batch 4096, complex64 I/O",
     "mfma_fp16split_c32h": "DFT-as-GEMM FP16 hi/lo, N=4096, batch 4096, complex-FP16 I/O",
     "sar_fp16": "range-Doppler 4096 x 4096 complex FP16, RCMC on, FP32 arithmetic",
+    "sar_fp16_config5": "range-Doppler 4096 x 4096 complex FP16, no RCMC (config 5), FP32 arithmetic",
     "convolve": "fused convolution 512 x 65536, K = 1023, N = 4096, forward + adjoint",
 }
 
@@ -593,13 +594,14 @@ def run_probe(name, torch, m, launches=4):
         for _ in range(launches):
             p.execute(x, y, 4096)
         done.append(p.kernel_name)
-    elif name == "sar_fp16":
-        rda = m.RangeDoppler(m.RdaParams.defaults(n_range=4096, n_azimuth=4096, io=1, rcmc=1))
+    elif name.startswith("sar_fp16"):
+        rcmc = 0 if name.endswith("config5") else 1
+        rda = m.RangeDoppler(m.RdaParams.defaults(n_range=4096, n_azimuth=4096, io=1, rcmc=rcmc))
         raw = (torch.randn((4096, 8192), generator=g, device="cuda") * 0.1).half()
         img = torch.empty_like(raw)
         for _ in range(launches):
             rda.process(raw, img)
-        done.append("k_rows / k_corner_turn / k_rcmc")
+        done.append("k_rows_stream / k_corner_turn_h16" + (" / k_rcmc" if rcmc else ""))
     elif name == "convolve":
         K, S, cnt = 1023, 65536, 512
         h = torch.randn(K, generator=g, device="cuda") / np.sqrt(K)
