@@ -54,6 +54,22 @@ def test_rda_small_vs_fp64(gpu, oracle, io, bound, rcmc):
         assert abs(ri - 3 + m[0] - rb) <= 1.0 and abs(ai - 8 + m[1] - ab) <= 4.0
 
 
+@pytest.mark.parametrize("n_range,n_azimuth", [(4096, 512), (1024, 4096), (4096, 2048)])
+@pytest.mark.parametrize("rcmc", [0, 1])
+def test_rda_n4096_row_forms(gpu, oracle, n_range, n_azimuth, rcmc):
+    """4096-point rows as one workgroup per row (row counts up to the
+    resident workgroups: 512 range rows, 1024 azimuth rows) and as the
+    persistent form that prefetches the next row (2048 range rows, two per
+    workgroup), complex64 data, against the FP64 pipeline."""
+    p = oracle.sar_default_params()
+    p.update(n_range=n_range, n_azimuth=n_azimuth)
+    tg = oracle.sar_targets(p, 6, 11)
+    got, ref, _ = run_rda(gpu, oracle, p, tg, 0, rcmc=rcmc)
+    err = oracle.rel_l2(got, ref)
+    print(f"{n_range}x{n_azimuth} rcmc={rcmc} rel_l2={err:.3e}")
+    assert err < 1e-5
+
+
 @pytest.mark.parametrize("rcmc", [0, 1])
 def test_rda_reference_formula_matches_oracle(gpu, oracle, rcmc):
     p = oracle.sar_default_params()
