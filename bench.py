@@ -475,6 +475,32 @@ def extra_sar(m, torch, n=4096):
     return out
 
 
+def extra_correlate(m, torch, count=4096, n=4096):
+    """Correlator.cross (Correlate.swift:27-61) batched on the device: 4096
+    pairs of real signals of 4096 samples, N = 4096 circular correlations in
+    one launch; accuracy against numpy's FP64 IFFT(A conj(B)) on a sample."""
+    g = torch.Generator(device="cuda").manual_seed(12)
+    a = torch.randn((count, n), generator=g, device="cuda")
+    b = torch.randn((count, n), generator=g, device="cuda")
+    r = torch.empty((count, n), device="cuda")
+    c = m.Correlator(n)
+    ms = time_fn(lambda: c.cross_device(a, n, b, n, count, r), 20, 4, torch)
+    c.cross_device(a, n, b, n, count, r)
+    torch.cuda.synchronize()
+    idx = list(range(0, count, 64))
+    a64 = a[idx].cpu().numpy().astype(np.float64)
+    b64 = b[idx].cpu().numpy().astype(np.float64)
+    ref = np.real(np.fft.ifft(np.fft.fft(a64, axis=1) * np.conj(np.fft.fft(b64, axis=1)), axis=1))
+    got = r[idx].cpu().numpy()
+    c.close()
+    nbytes = 4 * count * (n + n + n)  # a and b read, r written
+    return {"pairs": count, "samples": n, "fft": n, "ms": ms, "correlations_per_s": count / (ms * 1e-3),
+            "gbs_alg": nbytes / (ms * 1e-3) / 1e9,
+            "roofline_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "accuracy_vs_fp64": {"rel_l2": rel_l2(got, ref), "max_abs": max_abs(got, ref),
+                                 "sample": f"{len(idx)} of the {count} correlations"}}
+
+
 def extra_xspec(m, torch, c=8, s=1 << 20, n=1024, hop=512):
     """Cross-spectral analysis (PSD.crossSpectral): 8 channels x 2^20 samples,
     Hann window, N = 1024, hop 512 (2047 frames, 28 pairs)."""
@@ -782,6 +808,7 @@ def main():
                         ("convolve_512x65536_k1023",
                          lambda: extra_convolve(m, torch, cpu=not a.no_cpu)),
                         ("sar_4096x4096_fp16", lambda: extra_sar(m, torch)),
+                        ("correlate_4096x4096", lambda: extra_correlate(m, torch)),
                         ("cross_spectral_8ch", lambda: extra_xspec(m, torch)),
                         ("stft_welch_2p24", lambda: extra_stft(m, torch)),
                         ("real_fft_4096x4096", lambda: extra_real(m, torch))):

@@ -238,6 +238,33 @@ def test_correlator(gpu, oracle):
         assert int(np.argmax(ac)) == 0
 
 
+@pytest.mark.parametrize("count", [300, 2500])
+def test_correlator_device_batch_4096(gpu, oracle, count):
+    """Batched device correlations at N = 4096 (fewer and more pairs than
+    resident workgroups), a truncated (n + 37 samples) and b zero-padded
+    (n / 3), against the FP64 lag sum."""
+    import torch
+    n = 4096
+    na, nb = n + 37, n // 3
+    rng = np.random.default_rng(31)
+    a = rng.standard_normal((count, na)).astype(np.float32)
+    b = rng.standard_normal((count, nb)).astype(np.float32)
+    c = gpu.Correlator(n)
+    out = torch.empty((count, n), device="cuda")
+    c.cross_device(torch.from_numpy(a).cuda(), na, torch.from_numpy(b).cuda(), nb, count, out)
+    torch.cuda.synchronize()
+    r = out.cpu().numpy()
+    for i in (0, 1, count // 2, count - 1):
+        assert oracle.rel_l2(r[i], oracle.f64_correlate(a[i], b[i], n)) < 1e-6, i
+    # every row against the host FFT in FP64 (truncate / zero-pad, IFFT(A conj(B)))
+    ap = a[:, :n].astype(np.float64)
+    bp = np.zeros((count, n))
+    bp[:, :nb] = b
+    ref = np.real(np.fft.ifft(np.fft.fft(ap, axis=1) * np.conj(np.fft.fft(bp, axis=1)), axis=1))
+    assert oracle.rel_l2(r, ref) < 1e-6
+    c.close()
+
+
 def test_correlator_lag_convention(gpu):
     # r[m] = sum_n a[n+m] b[n]: a delayed copy of b peaks at the delay
     n = 1024
