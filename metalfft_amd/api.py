@@ -330,6 +330,10 @@ class PrecisionFFT:
     def kernel_name(self) -> str:
         return _lib.load().mifft_plan_kernel_name(self._plan).decode()
 
+    def select_kernel(self, name: str) -> None:
+        """Pins one of the plan's DFT-as-GEMM forms by name (mifft_plan_select_kernel)."""
+        _lib.check(_lib.load().mifft_plan_select_kernel(self._plan, name.encode()))
+
     def execute(self, src, dst, batch: int, direction: int = FORWARD, stream=None) -> None:
         batch = _count(batch, "batch")
         kinds = C32H if self.io == self.C32H else C64
