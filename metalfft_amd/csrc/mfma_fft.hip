@@ -259,8 +259,26 @@ k_fft4096_mfma(const typename IO::T* in, typename IO::T* out, const float2* __re
             ds_barrier();                   // every wave's DMA landed; previous transform done
         }
         // the transform as it sits in LDS (staged) or in HBM: 16 lanes of a
-        // row read 16 consecutive elements
-        const typename IO::T* src = STAGE ? static_cast<const typename IO::T*>(stage) : in + tr * N;
+        // row read 16 consecutive elements.  One-shot radix-64: the transform
+        // is first copied HBM -> LDS through registers with lane-contiguous
+        // loads (a wave reads 64 consecutive elements per instruction, where
+        // the operand layout would read four runs of 16 elements 1024 apart)
+        // into the exchange buffer, which pass 0 overwrites only after the
+        // max-reduction barrier below
+        constexpr bool VIA_LDS = !STAGE && R64;
+        if constexpr (VIA_LDS) {
+            typename IO::T x[16];
+            const typename IO::T* g = in + tr * N;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) x[m] = g[tid + 256 * m];
+            typename IO::T* st = reinterpret_cast<typename IO::T*>(lds_own);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) st[tid + 256 * m] = x[m];
+            ds_barrier();
+        }
+        const typename IO::T* src = STAGE     ? static_cast<const typename IO::T*>(stage)
+                                    : VIA_LDS ? reinterpret_cast<const typename IO::T*>(lds_own)
+                                              : in + tr * N;
         float2 v[4][4];  // [tile][i]: x_{4g+i} of butterfly 16*(4w+tile) + c
         float m = 0.f;
 #pragma unroll
