@@ -257,41 +257,41 @@ __device__ __forceinline__ void stockham_regs(float2 (&v)[E], float2* buf, int t
         float2 x[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) x[j] = v[i + B * j];
+        // twiddles x[j] *= w[j] (j >= 1) and the DFT-R (dft_tw: at R = 16
+        // the products fold into the butterfly's first adds)
         if constexpr (SD > 1) {
             const int pos = (t + i * T) & (SD - 1);
             if constexpr (PRE == 1) {  // register twiddles of pass P (PreTwiddles)
-#pragma unroll
-                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], pre[P - 1][i * R + j]);
+                dft_tw<R>(x, &pre[P - 1][i * R]);
             } else if constexpr (PRE == 2) {  // register table entries (PreBases)
                 float2 w[R];
                 powers_from_bases<R>(w, &pre[P - 1][i * twiddle_bases(R)]);
-#pragma unroll
-                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
+                dft_tw<R>(x, w);
             } else if constexpr (PRE == 3) {  // packed bases row, powers derived
                 constexpr int off = N + PassTable<N, E, 0>::offset(P);
                 float2 b[4];
                 load_row<(twiddle_bases(R) + 1) / 2>(b, tw + off + pos * 4);
                 float2 w[R];
                 powers_from_bases<R>(w, b);
-#pragma unroll
-                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
+                dft_tw<R>(x, w);
             } else if constexpr (PRE == 4) {  // packed powers row
                 constexpr int off = N + PassTable<N, E, 1>::offset(P);
                 float2 w[R];
                 load_row<R / 2>(w, tw + off + pos * R);
-#pragma unroll
-                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
+                dft_tw<R>(x, w);
             } else if constexpr (TW4) {
                 float2 w[R];
                 twiddle_powers<R>(w, tw, pos * TWS);
-#pragma unroll
-                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], w[j]);
+                dft_tw<R>(x, w);
             } else {
+                float2 w[R];
 #pragma unroll
-                for (int j = 1; j < R; ++j) x[j] = cmul(x[j], tw[j * pos * TWS]);
+                for (int j = 1; j < R; ++j) w[j] = tw[j * pos * TWS];
+                dft_tw<R>(x, w);
             }
+        } else {
+            dft<R>(x);
         }
-        dft<R>(x);
 #pragma unroll
         for (int j = 0; j < R; ++j) v[i + B * j] = x[j];
     }
@@ -645,10 +645,10 @@ __device__ __forceinline__ void stockham_half(float2 (&v)[E], float2* buf, int t
         if constexpr (P > 0) {
             float2 wp[R];
             powers_from_bases<R>(wp, &w[4 * i]);
-#pragma unroll
-            for (int j = 1; j < R; ++j) x[j] = cmul(x[j], wp[j]);
+            dft_tw<R>(x, wp);
+        } else {
+            dft<R>(x);
         }
-        dft<R>(x);
 #pragma unroll
         for (int j = 0; j < R; ++j) v[i + B * j] = x[j];
     }
