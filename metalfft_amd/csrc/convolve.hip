@@ -421,7 +421,7 @@ k_sum_rows64(const float* __restrict__ in, float* __restrict__ out, int64_t rows
 using namespace fp16;
 
 template <int PM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PM == kConvStorage ? 4 : 1, 8)))
 k_conv_fwd_h(const float* __restrict__ x, float* __restrict__ y, const float2* __restrict__ H,
              const float2* __restrict__ tw, ConvGeom g) {
     constexpr int N = 4096, T = 256;
