@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
 // the TPW slots are then added through LDS in a fixed order and written as
 // partial[j * cols + c * (N/2 + 1) + k].
 template <int N, bool WIN, bool PAIR>
-__global__ void __launch_bounds__(RCfg<N / 2>::threads)
+__global__ void __launch_bounds__(RCfg<N / 2>::threads) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_real_power(const float* __restrict__ sig, int64_t samples, int64_t hop, int64_t frames, int64_t wpc,
              int64_t iters, const float* __restrict__ window, float* __restrict__ partial, int64_t cols,
              const float2* __restrict__ tab, const float2* __restrict__ roots) {

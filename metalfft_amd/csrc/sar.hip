@@ -327,7 +327,7 @@ k_rows_stream(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const
 // divides by N only after the inverse and overflows FP16 for radar data
 // (its own "risk: overflow" note, fft_sar_fused_fp16.metal:18).
 template <int PM, int IO, class F, int MODE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_rows_h(const typename SarIo<IO>::T* in, typename SarIo<IO>::T* out, const float2* __restrict__ tw,
          F filt, float scale) {
     using namespace fp16;
