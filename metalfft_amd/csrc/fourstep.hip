@@ -84,14 +84,14 @@ struct FsTile {
 // Thread tid moves elements tid + THREADS q, q < 16: consecutive lanes take
 // consecutive columns of one row, so each group of C lanes reads 8 C
 // contiguous bytes.
-template <int L, int C, bool CONJ>
+template <int L, int C, bool CONJ, int LP = 0>
 __device__ __forceinline__ void stage_in(float2* lds, const float2* src, int ld, int tid) {
     using Tl = FsTile<L, C>;
     float2 x[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int f = tid + Tl::THREADS * q;
-        x[q] = gload<true>(src + (f % C) + (int64_t)(f / C) * ld);
+        x[q] = gload_p<LP>(src + (f % C) + (int64_t)(f / C) * ld);
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -102,7 +102,7 @@ __device__ __forceinline__ void stage_in(float2* lds, const float2* src, int ld,
 
 // Column c, row r of the tile -> dst[c + r * ld] (the transpose of
 // stage_in), conj and scale by 1/n for the inverse.
-template <int L, int C, bool INV, int NTOT>
+template <int L, int C, bool INV, int NTOT, int SP = 0>
 __device__ __forceinline__ void stage_out(const float2* lds, float2* dst, int ld, int tid) {
     using Tl = FsTile<L, C>;
 #pragma unroll
@@ -113,7 +113,7 @@ __device__ __forceinline__ void stage_out(const float2* lds, float2* dst, int ld
             constexpr float inv_n = 1.0f / (float)NTOT;
             y = make_float2(y.x * inv_n, -y.y * inv_n);
         }
-        gstore<true>(dst + (f % C) + (int64_t)(f / C) * ld, y);
+        gstore_p<SP>(dst + (f % C) + (int64_t)(f / C) * ld, y);
     }
 }
 
@@ -130,8 +130,9 @@ __device__ __forceinline__ void column_fft(float2 (&v)[16], float2* buf, int t,
 }
 
 // Pass A: columns n1 in [c0, c0 + CA) of transform tr; y row n1 = the
-// N2-point transform of column n1 times W_N^{n1 k2}.
-template <int N1, int N2, int CA, bool INV>
+// N2-point transform of column n1 times W_N^{n1 k2}, stored under cache
+// policy YP (gstore_p).
+template <int N1, int N2, int CA, bool INV, int YP>
 __global__ void __launch_bounds__((N2 / 16) * CA)
 k_fourstep_cols(const float2* in, float2* y, const float2* __restrict__ tw, int64_t batch) {
     using Tl = FsTile<N2, CA>;
@@ -158,17 +159,18 @@ k_fourstep_cols(const float2* in, float2* y, const float2* __restrict__ tw, int6
     // reduction mod N
     const int ea = n1 * t;
     const float2 wa = cmul(hi[ea >> S], lo[ea & ((1 << S) - 1)]);
-    gstore<true>(dst + t, cmul(v[0], wa));
+    gstore_p<YP>(dst + t, cmul(v[0], wa));
 #pragma unroll
     for (int m = 1; m < 16; ++m) {
         const int eb = n1 * (T * m);
         const float2 w = cmul(wa, cmul(hi[eb >> S], lo[eb & ((1 << S) - 1)]));
-        gstore<true>(dst + t + T * m, cmul(v[m], w));
+        gstore_p<YP>(dst + t + T * m, cmul(v[m], w));
     }
 }
 
-// Pass B: columns k2 in [k0, k0 + CB) of y (N1 rows of N2) -> X[k2 + N2 k1].
-template <int N1, int N2, int CB, bool INV>
+// Pass B: columns k2 in [k0, k0 + CB) of y (N1 rows of N2) -> X[k2 + N2 k1];
+// y read under policy YP, X stored under OP.
+template <int N1, int N2, int CB, bool INV, int YP, int OP>
 __global__ void __launch_bounds__((N1 / 16) * CB)
 k_fourstep_rows(const float2* y, float2* out, const float2* __restrict__ tw, int64_t batch) {
     using Tl = FsTile<N1, CB>;
@@ -179,7 +181,7 @@ k_fourstep_rows(const float2* y, float2* out, const float2* __restrict__ tw, int
     const int64_t tr = blockIdx.x / TILES;
     const int k0 = (int)(blockIdx.x % TILES) * CB;
     const int tid = threadIdx.x;
-    stage_in<N1, CB, false>(lds, y + tr * N + k0, N2, tid);
+    stage_in<N1, CB, false, YP>(lds, y + tr * N + k0, N2, tid);
     const int local = tid / T, t = tid % T;
     float2* buf = lds + local * Tl::LEN;
     float2 v[16];
@@ -188,10 +190,10 @@ k_fourstep_rows(const float2* y, float2* out, const float2* __restrict__ tw, int
 #pragma unroll
     for (int m = 0; m < 16; ++m) buf[LdsSwz::idx(t + T * m)] = v[m];
     lds_barrier();
-    stage_out<N1, CB, INV, N>(lds, out + tr * N + k0, N2, tid);
+    stage_out<N1, CB, INV, N, OP>(lds, out + tr * N + k0, N2, tid);
 }
 
-template <int N1, int N2, int CA, int CB>
+template <int N1, int N2, int CA, int CB, int YP, int OP>
 hipError_t launch_fs(const float2* in, float2* out, float2* y, const float2* tw, int64_t batch,
                      bool inverse, hipStream_t st) {
     if (batch <= 0) return hipSuccess;
@@ -199,22 +201,32 @@ hipError_t launch_fs(const float2* in, float2* out, float2* y, const float2* tw,
     if (ga > 0x7fffffffLL || gb > 0x7fffffffLL) return hipErrorInvalidValue;
     constexpr int ta = FsTile<N2, CA>::THREADS, tb = FsTile<N1, CB>::THREADS;
     if (inverse) {
-        hipLaunchKernelGGL((k_fourstep_cols<N1, N2, CA, true>), dim3((unsigned)ga), dim3(ta), 0, st, in, y,
+        hipLaunchKernelGGL((k_fourstep_cols<N1, N2, CA, true, YP>), dim3((unsigned)ga), dim3(ta), 0, st, in, y,
                            tw, batch);
-        hipLaunchKernelGGL((k_fourstep_rows<N1, N2, CB, true>), dim3((unsigned)gb), dim3(tb), 0, st, y, out,
-                           tw, batch);
+        hipLaunchKernelGGL((k_fourstep_rows<N1, N2, CB, true, YP, OP>), dim3((unsigned)gb), dim3(tb), 0, st, y,
+                           out, tw, batch);
     } else {
-        hipLaunchKernelGGL((k_fourstep_cols<N1, N2, CA, false>), dim3((unsigned)ga), dim3(ta), 0, st, in, y,
-                           tw, batch);
-        hipLaunchKernelGGL((k_fourstep_rows<N1, N2, CB, false>), dim3((unsigned)gb), dim3(tb), 0, st, y, out,
-                           tw, batch);
+        hipLaunchKernelGGL((k_fourstep_cols<N1, N2, CA, false, YP>), dim3((unsigned)ga), dim3(ta), 0, st, in,
+                           y, tw, batch);
+        hipLaunchKernelGGL((k_fourstep_rows<N1, N2, CB, false, YP, OP>), dim3((unsigned)gb), dim3(tb), 0, st, y,
+                           out, tw, batch);
     }
     return hipGetLastError();
 }
 
 #define MIFFT_FS(N1, N2, CA, CB)                                                      \
-    {N1 * N2, N1, N2, CA, CB, &launch_fs<N1, N2, CA, CB>,                             \
+    {N1 * N2, N1, N2, CA, CB, &launch_fs<N1, N2, CA, CB, 0, 0>,                       \
      "fourstep_" #N1 "x" #N2 "_c" #CA "_c" #CB}
+#define MIFFT_FSP(N1, N2, CA, CB, YP, OP)                                             \
+    {N1 * N2, N1, N2, CA, CB, &launch_fs<N1, N2, CA, CB, YP, OP>,                     \
+     "fourstep_" #N1 "x" #N2 "_c" #CA "_c" #CB "_y" #YP "o" #OP}
+// The same shapes with the intermediate stored (and read back) under the
+// default or the system-coherent policy instead of the streaming one, and
+// the output streaming or system-coherent: pass B then finds part of y in
+// the caches (at 2^20, y2o2: 133 -> 121 us per 128 MiB)
+#define MIFFT_FSPS(N1, N2, CA, CB)                                                    \
+    MIFFT_FSP(N1, N2, CA, CB, 1, 0), MIFFT_FSP(N1, N2, CA, CB, 2, 0),                 \
+    MIFFT_FSP(N1, N2, CA, CB, 1, 2), MIFFT_FSP(N1, N2, CA, CB, 2, 2)
 
 // Default first for each length (find_fourstep); the autotuner times all.
 const FourStepVariant kFourStep[] = {
@@ -224,6 +236,8 @@ const FourStepVariant kFourStep[] = {
     MIFFT_FS(512, 512, 16, 16),   MIFFT_FS(512, 512, 8, 8),
     MIFFT_FS(1024, 512, 16, 16),  MIFFT_FS(512, 1024, 16, 16),  MIFFT_FS(1024, 512, 16, 8),
     MIFFT_FS(1024, 1024, 16, 16), MIFFT_FS(1024, 1024, 8, 8),
+    MIFFT_FSPS(256, 256, 32, 32), MIFFT_FSPS(512, 256, 16, 16), MIFFT_FSPS(512, 512, 16, 16),
+    MIFFT_FSPS(512, 1024, 16, 16), MIFFT_FSPS(1024, 1024, 16, 16),
 };
 #undef MIFFT_FS
 

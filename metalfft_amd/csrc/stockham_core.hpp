@@ -138,6 +138,25 @@ __device__ __forceinline__ void gstore(float2* p, float2 v) {
     }
 }
 
+// Cache policies of global stores and loads, P: 0 non-temporal (streaming),
+// 1 the default policy, 2 system-coherent stores (sc0 sc1; loads as 1).
+template <int P>
+__device__ __forceinline__ float2 gload_p(const float2* p) {
+    return gload<P == 0>(p);
+}
+
+template <int P>
+__device__ __forceinline__ void gstore_p(float2* p, float2 v) {
+    if constexpr (P == 2) {
+        const unsigned long long u =
+            ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), u, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+        gstore<P == 0>(p, v);
+    }
+}
+
 // ------------------------------------------------------------ twiddles
 // Number of table entries a radix-R butterfly reads: W^k, W^2k, W^4k, W^8k
 // for R = 16 (log2 R of them).
