@@ -220,7 +220,7 @@ hipError_t launch_fs(const float2* in, float2* out, float2* y, const float2* tw,
 #define MIFFT_FSP(N1, N2, CA, CB, YP, OP)                                             \
     {N1 * N2, N1, N2, CA, CB, &launch_fs<N1, N2, CA, CB, YP, OP>,                     \
      "fourstep_" #N1 "x" #N2 "_c" #CA "_c" #CB "_y" #YP "o" #OP}
-// The same shapes with the intermediate stored (and read back) under the
+// Every shape again with the intermediate stored (and read back) under the
 // default or the system-coherent policy instead of the streaming one, and
 // the output streaming or system-coherent: pass B then finds part of y in
 // the caches (at 2^20, y2o2: 133 -> 121 us per 128 MiB)
@@ -236,8 +236,12 @@ const FourStepVariant kFourStep[] = {
     MIFFT_FS(512, 512, 16, 16),   MIFFT_FS(512, 512, 8, 8),
     MIFFT_FS(1024, 512, 16, 16),  MIFFT_FS(512, 1024, 16, 16),  MIFFT_FS(1024, 512, 16, 8),
     MIFFT_FS(1024, 1024, 16, 16), MIFFT_FS(1024, 1024, 8, 8),
-    MIFFT_FSPS(256, 256, 32, 32), MIFFT_FSPS(512, 256, 16, 16), MIFFT_FSPS(512, 512, 16, 16),
-    MIFFT_FSPS(512, 1024, 16, 16), MIFFT_FSPS(1024, 1024, 16, 16),
+    MIFFT_FSPS(256, 128, 32, 16), MIFFT_FSPS(128, 256, 16, 32), MIFFT_FSPS(256, 128, 16, 16),
+    MIFFT_FSPS(256, 256, 16, 16), MIFFT_FSPS(256, 256, 32, 32),
+    MIFFT_FSPS(512, 256, 16, 16), MIFFT_FSPS(256, 512, 16, 16), MIFFT_FSPS(512, 256, 32, 16),
+    MIFFT_FSPS(512, 512, 16, 16), MIFFT_FSPS(512, 512, 8, 8),
+    MIFFT_FSPS(1024, 512, 16, 16), MIFFT_FSPS(512, 1024, 16, 16), MIFFT_FSPS(1024, 512, 16, 8),
+    MIFFT_FSPS(1024, 1024, 16, 16), MIFFT_FSPS(1024, 1024, 8, 8),
 };
 #undef MIFFT_FS
 
