@@ -46,6 +46,35 @@ def test_every_fourstep_variant(gpu, oracle, n):
         assert oracle.rel_l2(back.cpu().numpy(), oracle.cpu_fft(cpu, inverse=True, threads=8)) < 1e-6
 
 
+@pytest.mark.parametrize("n", FOURSTEP_SIZES[1:])
+def test_cache_policies_same_bits(gpu, n):
+    """A shape compiled with another cache policy for its intermediate and
+    output (suffix _y<p>o<q>) runs the same arithmetic: forward and inverse
+    equal the streaming-policy shape bit for bit, also in place."""
+    import re
+    import torch
+    x = torch.from_numpy(uniform_c64((2, n), n + 5)).cuda()
+    fft = gpu.MetalFFT(n)
+    names = [s for s in gpu.kernel_variants(n) if s.startswith("fourstep_")]
+    policy = [s for s in names if re.search(r"_y\d+o\d+$", s)]
+    assert policy
+    for name in policy:
+        base = re.sub(r"_y\d+o\d+$", "", name)
+        assert base in names, name
+        outs = []
+        for k in (base, name):
+            fft.select_kernel(k)
+            print(f"fourstep n={n} variant={k}", flush=True)
+            y = torch.empty_like(x)
+            fft.execute(x, y, batch=2)
+            z = x.clone()
+            fft.execute(z, z, batch=2, direction=gpu.INVERSE)
+            torch.cuda.synchronize()
+            outs.append((y, z))
+        assert torch.equal(outs[0][0], outs[1][0]), name
+        assert torch.equal(outs[0][1], outs[1][1]), name
+
+
 @pytest.mark.parametrize("n", [32768, 1 << 20])
 def test_in_place_and_batch_single(gpu, n):
     import torch
