@@ -658,6 +658,8 @@ PROBES = {
     "sar_fp16": "range-Doppler 4096 x 4096 complex FP16, RCMC on, FP32 arithmetic",
     "sar_fp16_config5": "range-Doppler 4096 x 4096 complex FP16, no RCMC (config 5), FP32 arithmetic",
     "convolve": "fused convolution 512 x 65536, K = 1023, N = 4096, forward + adjoint",
+    "convolve_fp16": "fused convolution 512 x 65536, K = 1023, N = 4096, forward in the FP16 modes "
+                     "pure, storage, mixed",
 }
 
 
@@ -704,6 +706,16 @@ def run_probe(name, torch, m, launches=4):
             cv.apply_device(x, y, S, cnt)
             cv.backward_device(x, gy, gx, gh, S, cnt)
         done.append("k_conv_fwd / k_conv_bwd_x / k_conv_bwd_h / k_sum_rows64")
+    elif name == "convolve_fp16":
+        K, S, cnt = 1023, 65536, 512
+        h = (torch.randn(K, generator=g, device="cuda") / np.sqrt(K)).cpu().numpy()
+        x = torch.randn((cnt, S), generator=g, device="cuda")
+        y = torch.empty((cnt, S + K - 1), device="cuda")
+        for prec in ("float16Pure", "float16Storage", "float16Mixed"):
+            cv = m.FusedConvolver(h, 4096, getattr(m.Precision, prec))
+            for _ in range(launches):
+                cv.apply_device(x, y, S, cnt)
+        done.append("k_conv_fwd_h<1> / <2> / <3>")
     torch.cuda.synchronize()
     print(json.dumps({"probe": name, "workload": PROBES[name], "launches": launches, "kernels": done}))
 
