@@ -243,6 +243,8 @@ const FourStepVariant kFourStep[] = {
     MIFFT_FSPS(1024, 512, 16, 16), MIFFT_FSPS(512, 1024, 16, 16), MIFFT_FSPS(1024, 512, 16, 8),
     MIFFT_FSPS(1024, 1024, 16, 16), MIFFT_FSPS(1024, 1024, 8, 8),
 };
+#undef MIFFT_FSPS
+#undef MIFFT_FSP
 #undef MIFFT_FS
 
 }  // namespace
