@@ -117,8 +117,7 @@ typedef struct mifft_plan mifft_plan;
  * Creation allocates the plan's twiddle tables and stream and, for FP32
  * plans, the first creation of a length in a process times the compiled
  * kernel variants of that length (about 1 GiB of temporary device memory
- * for 0.1-0.5 s, longest for the four-step lengths; MIFFT_AUTOTUNE=0
- * skips it).  The choice is
+ * for some tens of milliseconds; MIFFT_AUTOTUNE=0 skips it).  The choice is
  * kept in memory; it is written to disk only when MIFFT_AUTOTUNE_CACHE
  * names a file. */
 MIFFT_API mifft_status mifft_plan_create(int64_t n, mifft_plan **plan);

@@ -145,19 +145,15 @@ bool tune_log() {
 // four times the 256 MiB Infinity Cache), as a pipeline over fresh batches
 // runs: with only two sets, kernels whose loads allocate in that cache were
 // timed up to 15 % faster than they run on fresh data.  Candidates are timed
-// round-robin (five rounds of 32 launches each, after eight warm-up
-// launches), so a drift of the clocks during tuning does not favour
-// whichever ran first, and each is scored by its median round.  Rounds of
-// eight launches scored by the best one favoured short bursts: at 8192 a
-// form that ties over eight launches runs 6 % slower over hundreds (52.7
-// against 49.5 us per 128 MiB), which rounds of 32 show (25.1 against 24.1
-// ns per transform).  A preferred (stateless,
+// round-robin (five rounds of eight launches each, after eight warm-up
+// launches) and each keeps its best round, so a drift of the clocks during
+// tuning does not favour whichever ran first.  A preferred (stateless,
 // one-shot) candidate within 3 % of the fastest is kept instead: persistent
 // and DMA-staged forms trade a per-launch prologue for overlap, so their
 // time per transform moves more with batch size and launch mode.
 int time_candidates(int n, size_t elem_bytes, const std::vector<Candidate>& cand, hipStream_t stream,
                     const char* what, double* best_ns) {
-    constexpr int kSets = 4, kRounds = 5, kReps = 32;
+    constexpr int kSets = 4, kRounds = 5, kReps = 8;
     const int64_t batch = std::max<int64_t>(1, (int64_t(1) << 24) / n);  // per launch
     const int64_t set = (int64_t)n * batch, total = set * kSets;
     char *a = nullptr, *b = nullptr;
@@ -186,7 +182,6 @@ int time_candidates(int n, size_t elem_bytes, const std::vector<Candidate>& cand
         if (run(cand[q], kReps)) ok.push_back((int)q);
     (void)hipGetLastError();
     std::vector<double> t(cand.size(), 1e30);
-    std::vector<std::vector<double>> rounds(cand.size());
     for (int round = 0; round < kRounds; ++round) {
         for (int q : ok) {
             (void)hipEventRecord(e0, stream);
@@ -195,13 +190,8 @@ int time_candidates(int n, size_t elem_bytes, const std::vector<Candidate>& cand
             (void)hipEventSynchronize(e1);
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            rounds[q].push_back(1e6 * ms / kReps / batch);
+            t[q] = std::min(t[q], 1e6 * ms / kReps / batch);
         }
-    }
-    for (int q : ok) {
-        std::vector<double>& r = rounds[q];
-        std::sort(r.begin(), r.end());
-        t[q] = r[r.size() / 2];
     }
     if (tune_log())
         for (int q : ok) std::fprintf(stderr, "mifft autotune %s n=%d %s %.2f ns/transform\n", what, n, cand[q].name, t[q]);

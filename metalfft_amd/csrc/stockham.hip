@@ -187,7 +187,10 @@ static const StockhamVariant kVariants[] = {
     MIFFT_VD(8192, 8), MIFFT_VD(8192, 16),
     MIFFT_V(16384, 16), MIFFT_VS(16384, 16), MIFFT_VP(16384, 16),
     MIFFT_VT(64),      MIFFT_VT(128),     MIFFT_VT(256),      MIFFT_VT(512),      MIFFT_VT(1024),
-    MIFFT_VT(2048),    MIFFT_VT(4096),    MIFFT_VT(8192),     MIFFT_VT(16384),
+    // not at 8192: there it ties with the half-LDS form over the tuner's
+    // short bursts of launches and runs 6 % slower over a sustained stream
+    // (52.7-53.0 against 49.4-50.0 us per 128 MiB), so it was sometimes chosen
+    MIFFT_VT(2048),    MIFFT_VT(4096),    MIFFT_VT(16384),
     MIFFT_VPT(1024),   MIFFT_VPT(2048),   MIFFT_VPT(4096),    MIFFT_VPT(8192),
     MIFFT_VP3(16384),
     MIFFT_VH(2048, 4),  MIFFT_VH(4096, 4), MIFFT_VH(8192, 4),
