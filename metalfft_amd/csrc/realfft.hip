@@ -23,6 +23,11 @@
 namespace mifft {
 namespace {
 
+// Exchange layout of the half-length transforms: XOR-swizzled, no padding,
+// so two 2048-point transforms take 32 KiB (five workgroups per CU; the
+// padded layout's 34 KiB allowed four).
+using RLds = LdsSwz;
+
 template <int M>
 struct RCfg {
     static constexpr int E = M >= 256 ? 16 : 8;
@@ -62,10 +67,10 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
     constexpr int M = N / 2;
     using C = RCfg<M>;
     constexpr int E = C::E, T = M / E;
-    __shared__ float2 lds[C::TPW * LdsPad::len(M)];
+    __shared__ float2 lds[C::TPW * RLds::len(M)];
     const int local = threadIdx.x / T;
     const int t = threadIdx.x % T;
-    float2* buf = lds + local * LdsPad::len(M);
+    float2* buf = lds + local * RLds::len(M);
     const int64_t g = (int64_t)blockIdx.x * C::TPW + local;
     const bool act = g < a.total;
     const int64_t gc = act ? g : 0;  // an in-range frame for the loads
@@ -75,10 +80,10 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
     for (int m = 0; m < E; ++m) v[m] = ld_pair<WIN, PAIR>(p, a.window, 2 * (t + T * m));  // whole frames
     RegTwiddles<M, E> rt;
     rt.load(a.tab, t);
-    rt.template fft<false>(v, buf, t, a.tab);
+    rt.template fft<false, RLds>(v, buf, t, a.tab);
     lds_barrier();
 #pragma unroll
-    for (int m = 0; m < E; ++m) buf[LdsPad::idx(t + T * m)] = v[m];
+    for (int m = 0; m < E; ++m) buf[RLds::idx(t + T * m)] = v[m];
     lds_barrier();
     const bool full = a.bins == N;
     float2* out = a.spec + gc * a.bins;
@@ -86,7 +91,7 @@ __global__ void __launch_bounds__(RCfg<N / 2>::threads) k_real_frames(RealFrames
     for (int m = 0; m < E; ++m) {
         const int k = t + T * m;
         const float2 z = v[m];
-        const float2 zc = conjf2(buf[LdsPad::idx((M - k) & (M - 1))]);
+        const float2 zc = conjf2(buf[RLds::idx((M - k) & (M - 1))]);
         const float2 e = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
         const float2 o = mul_mi(make_float2(0.5f * (z.x - zc.x), 0.5f * (z.y - zc.y)));
         const float2 wo = cmul(roots[k], o);
@@ -125,10 +130,10 @@ k_real_power(const float* __restrict__ sig, int64_t samples, int64_t hop, int64_
     constexpr int M = N / 2;
     using C = RCfg<M>;
     constexpr int E = C::E, T = M / E;
-    __shared__ float2 lds[C::TPW * LdsPad::len(M)];
+    __shared__ float2 lds[C::TPW * RLds::len(M)];
     const int local = threadIdx.x / T;
     const int t = threadIdx.x % T;
-    float2* buf = lds + local * LdsPad::len(M);
+    float2* buf = lds + local * RLds::len(M);
     const int64_t c = blockIdx.x / wpc, j = blockIdx.x % wpc;
     const float* base = sig + c * samples;
     float acc[E], acc_m = 0.f;
@@ -144,17 +149,17 @@ k_real_power(const float* __restrict__ sig, int64_t samples, int64_t hop, int64_
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = ld_pair<WIN, PAIR>(p, window, 2 * (tl + T * m));
-        rt.template fft<true>(v, buf, tl, tab);
+        rt.template fft<true, RLds>(v, buf, tl, tab);
         lds_barrier();
 #pragma unroll
-        for (int m = 0; m < E; ++m) buf[LdsPad::idx(tl + T * m)] = v[m];
+        for (int m = 0; m < E; ++m) buf[RLds::idx(tl + T * m)] = v[m];
         lds_barrier();
         const float keep = act ? 1.f : 0.f;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int k = tl + T * m;
             const float2 z = v[m];
-            const float2 zc = conjf2(buf[LdsPad::idx((M - k) & (M - 1))]);
+            const float2 zc = conjf2(buf[RLds::idx((M - k) & (M - 1))]);
             const float2 e = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
             const float2 o = mul_mi(make_float2(0.5f * (z.x - zc.x), 0.5f * (z.y - zc.y)));
             const float2 lo = cadd(e, cmul(roots[k], o));
@@ -186,10 +191,10 @@ k_real_inverse(const float2* __restrict__ y, int64_t in_stride, float* __restric
     constexpr int M = N / 2;
     using C = RCfg<M>;
     constexpr int E = C::E, T = M / E;
-    __shared__ float2 lds[C::TPW * LdsPad::len(M)];
+    __shared__ float2 lds[C::TPW * RLds::len(M)];
     const int local = threadIdx.x / T;
     const int t = threadIdx.x % T;
-    float2* buf = lds + local * LdsPad::len(M);
+    float2* buf = lds + local * RLds::len(M);
     const int64_t b = (int64_t)blockIdx.x * C::TPW + local;
     const bool act = b < batch;
     const float2* row = y + (act ? b : 0) * in_stride;
@@ -206,7 +211,7 @@ k_real_inverse(const float2* __restrict__ y, int64_t in_stride, float* __restric
     }
     RegTwiddles<M, E> rt;
     rt.load(tab, t);
-    rt.template fft<false>(v, buf, t, tab);
+    rt.template fft<false, RLds>(v, buf, t, tab);
     if (act) {
         constexpr float inv_m = 1.0f / (float)M;
         float* out = x + b * N;
