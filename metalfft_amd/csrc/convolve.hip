@@ -615,7 +615,10 @@ int conv_tpw_for(int n) {
 // A*conj(B) written as (:40-45), the conjugate-trick inverse, real part).
 // Here a and b share one FFT, z = a + i b, and are split with
 // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = -i (Z[k] - conj Z[-k]) / 2, the
-// mirrored bin read back through LDS.
+// mirrored bin read back through LDS.  P = A conj(B) is Hermitian (r is
+// real), so a transform slot takes two pairs and inverts Q = P1 + i P2 once:
+// r1 is the real part of IFFT(Q) and r2 its imaginary part, three
+// transforms per two correlations instead of four.
 template <int N, int E, int TPW>
 __global__ void __launch_bounds__((N / E) * TPW)
 k_correlate(const float* __restrict__ a, int64_t na, const float* __restrict__ b, int64_t nb,
@@ -625,50 +628,66 @@ k_correlate(const float* __restrict__ a, int64_t na, const float* __restrict__ b
     const int local = threadIdx.x / T;
     const int t = threadIdx.x % T;
     float2* buf = lds + local * LdsPad::len(N);
-    const int64_t item = (int64_t)blockIdx.x * TPW + local;
-    const bool active = item < count;
-    const float* as = a + (active ? item : 0) * na;
-    const float* bs = b + (active ? item : 0) * nb;
-    const Win A = window(as, na > 0 ? na : 1, 0, N), Bw = window(bs, nb > 0 ? nb : 1, 0, N);
-    float2 v[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) {  // branch-free loads (ld_win); an empty input reads as 0
-        const int n = t + T * m;
-        const float re = na > 0 ? ld_win(A, n) : 0.f;
-        const float im = nb > 0 ? ld_win(Bw, n) : 0.f;
-        v[m] = make_float2(active ? re : 0.f, active ? im : 0.f);
-    }
+    const int64_t item = (int64_t)blockIdx.x * TPW + local;  // pairs 2 item, 2 item + 1
     RegTwiddles<N, E> ct;
     ct.load(tw, t);
-    ct.template fft<false>(v, buf, t, tw);
-    lds_barrier();
+    float2 q[E];
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        const int64_t pi = 2 * item + h;
+        const bool active = pi < count;
+        const float* as = a + (active ? pi : 0) * na;
+        const float* bs = b + (active ? pi : 0) * nb;
+        const Win A = window(as, na > 0 ? na : 1, 0, N), Bw = window(bs, nb > 0 ? nb : 1, 0, N);
+        const int tl = fresh(t);  // index arithmetic redone per pair, not held
+        float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) buf[LdsPad::idx(t + T * m)] = v[m];
-    lds_barrier();
+        for (int m = 0; m < E; ++m) {  // branch-free loads (ld_win); an empty input reads as 0
+            const int n = tl + T * m;
+            const float re = na > 0 ? ld_win(A, n) : 0.f;
+            const float im = nb > 0 ? ld_win(Bw, n) : 0.f;
+            v[m] = make_float2(active ? re : 0.f, active ? im : 0.f);
+        }
+        ct.template fft<true>(v, buf, tl, tw);  // SYNC0: the first pair's split read the LDS
+        lds_barrier();
+#pragma unroll
+        for (int m = 0; m < E; ++m) buf[LdsPad::idx(tl + T * m)] = v[m];
+        lds_barrier();
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int k = tl + T * m;
+            const float2 zm = conjf2(buf[LdsPad::idx((N - k) & (N - 1))]);
+            const float2 A = make_float2(0.5f * (v[m].x + zm.x), 0.5f * (v[m].y + zm.y));
+            const float2 d = make_float2(0.5f * (v[m].x - zm.x), 0.5f * (v[m].y - zm.y));
+            const float2 B = mul_mi(d);
+            // A * conj(B) as (A.x B.x + A.y B.y, A.y B.x - A.x B.y)
+            const float2 P = make_float2(A.x * B.x + A.y * B.y, A.y * B.x - A.x * B.y);
+            q[m] = h == 0 ? P : make_float2(q[m].x - P.y, q[m].y + P.x);  // Q = P1 + i P2
+        }
+    }
+    // IFFT(Q) = conj(FFT(conj Q)) / N = r1 + i r2
+#pragma unroll
+    for (int m = 0; m < E; ++m) q[m] = conjf2(q[m]);
+    ct.template fft<true>(q, buf, t, tw);
+    const int64_t p0 = 2 * item;
+    if (p0 >= count) return;
+    const bool has1 = p0 + 1 < count;
+    constexpr float inv_n = 1.0f / (float)N;
+    float* r0 = r + p0 * N;
+    const int ts = fresh(t);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        const int k = t + T * m;
-        const float2 zm = conjf2(buf[LdsPad::idx((N - k) & (N - 1))]);
-        const float2 A = make_float2(0.5f * (v[m].x + zm.x), 0.5f * (v[m].y + zm.y));
-        const float2 d = make_float2(0.5f * (v[m].x - zm.x), 0.5f * (v[m].y - zm.y));
-        const float2 B = mul_mi(d);
-        // A * conj(B) as (A.x B.x + A.y B.y, A.y B.x - A.x B.y); conj for the inverse
-        const float2 P = make_float2(A.x * B.x + A.y * B.y, A.y * B.x - A.x * B.y);
-        v[m] = conjf2(P);
+        r0[ts + T * m] = q[m].x * inv_n;
+        if (has1) r0[N + ts + T * m] = -q[m].y * inv_n;
     }
-    ct.template fft<true>(v, buf, t, tw);
-    if (!active) return;
-    constexpr float inv_n = 1.0f / (float)N;
-    float* rs = r + item * N;
-#pragma unroll
-    for (int m = 0; m < E; ++m) rs[t + T * m] = v[m].x * inv_n;
 }
 
 template <int N>
 hipError_t corr_n(const float* a, int64_t na, const float* b, int64_t nb, float* r, const float2* tw,
                   int64_t count, hipStream_t st) {
     using C = ConvCfg<N>;
-    const int64_t blocks = (count + C::TPW - 1) / C::TPW;
+    const int64_t slots = (count + 1) / 2;  // two pairs per transform slot
+    const int64_t blocks = (slots + C::TPW - 1) / C::TPW;
     if (blocks <= 0) return hipSuccess;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_correlate<N, C::E, C::TPW>), dim3((unsigned)blocks), dim3(C::threads), 0, st,
