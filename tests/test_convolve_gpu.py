@@ -238,23 +238,25 @@ def test_correlator(gpu, oracle):
         assert int(np.argmax(ac)) == 0
 
 
-@pytest.mark.parametrize("count", [300, 2500])
-def test_correlator_device_batch_4096(gpu, oracle, count):
-    """Batched device correlations at N = 4096 (fewer and more pairs than
-    resident workgroups), a truncated (n + 37 samples) and b zero-padded
-    (n / 3), against the FP64 lag sum."""
+@pytest.mark.parametrize("n,count", [(4096, 300), (4096, 301), (4096, 2500), (256, 7), (64, 1)])
+def test_correlator_device_batch(gpu, oracle, n, count):
+    """Batched device correlations (fewer and more pairs than resident
+    workgroups; odd counts leave the last transform slot one pair, and at
+    256 and 64 points one workgroup holds several slots), a truncated
+    (n + 37 samples) and b zero-padded (n / 3), against the FP64 lag sum;
+    the row after the last pair is never written."""
     import torch
-    n = 4096
     na, nb = n + 37, n // 3
     rng = np.random.default_rng(31)
     a = rng.standard_normal((count, na)).astype(np.float32)
     b = rng.standard_normal((count, nb)).astype(np.float32)
     c = gpu.Correlator(n)
-    out = torch.empty((count, n), device="cuda")
+    out = torch.full((count + 1, n), float("nan"), device="cuda")
     c.cross_device(torch.from_numpy(a).cuda(), na, torch.from_numpy(b).cuda(), nb, count, out)
     torch.cuda.synchronize()
-    r = out.cpu().numpy()
-    for i in (0, 1, count // 2, count - 1):
+    assert torch.isnan(out[count]).all()
+    r = out[:count].cpu().numpy()
+    for i in sorted({0, 1, count // 2, count - 1} & set(range(count))):
         assert oracle.rel_l2(r[i], oracle.f64_correlate(a[i], b[i], n)) < 1e-6, i
     # every row against the host FFT in FP64 (truncate / zero-pad, IFFT(A conj(B)))
     ap = a[:, :n].astype(np.float64)
