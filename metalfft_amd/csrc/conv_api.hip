@@ -63,7 +63,7 @@ mifft_status mifft_convolver_create(const float* taps, int64_t k, int64_t fft_si
         mifft_convolver_destroy(cv);
         return stream_err();
     }
-    if ((s = cv->spectrum.reserve(8 * (size_t)fft_size)) != MIFFT_OK ||
+    if ((s = cv->spectrum.reserve(12 * (size_t)fft_size)) != MIFFT_OK ||  // H, then H in FP16
         (s = cv->packed.reserve(8 * (size_t)fft_size)) != MIFFT_OK ||
         (s = cv->hx.reserve(4 * (size_t)k)) != MIFFT_OK) {
         mifft_convolver_destroy(cv);

@@ -423,7 +423,7 @@ using namespace fp16;
 template <int PM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PM == kConvStorage ? 4 : 1, 8)))
 k_conv_fwd_h(const float* __restrict__ x, float* __restrict__ y, const float2* __restrict__ H,
-             const float2* __restrict__ tw, ConvGeom g) {
+             const h2* __restrict__ H16, const float2* __restrict__ tw, ConvGeom g) {
     constexpr int N = 4096, T = 256;
     __shared__ h2 lds[LdsPad::len(N)];
     const int t = threadIdx.x;
@@ -469,10 +469,9 @@ k_conv_fwd_h(const float* __restrict__ x, float* __restrict__ y, const float2* _
     // for the inverse transform
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-        const float2 hm = H[t + T * m];
-        if constexpr (PM == kConvPure) v[m] = hconj(hmul(v[m], to_h2(hm)));
-        else if constexpr (PM == kConvMixed) v[m] = to_f2(hconj(hmul(to_h2(v[m]), to_h2(hm))));
-        else v[m] = conjf2(round_h(cmul(round_h(v[m]), hm)));
+        if constexpr (PM == kConvPure) v[m] = hconj(hmul(v[m], H16[t + T * m]));
+        else if constexpr (PM == kConvMixed) v[m] = to_f2(hconj(hmul(to_h2(v[m]), H16[t + T * m])));
+        else v[m] = conjf2(round_h(cmul(round_h(v[m]), H[t + T * m])));
     }
     pb.opaque();
     conv_passes_h<PM, 0, true>(v, lds, t, pb.w);
@@ -508,18 +507,19 @@ hipError_t conv_fwd_fp16(int precision, const float* x, float* y, const float2* 
     const int64_t blocks = g.count * g.npair;
     if (blocks <= 0) return hipSuccess;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    const h2* H16 = reinterpret_cast<const h2*>(H + 4096);  // after H (convolver_spectrum)
     switch (precision) {
         case kConvPure:
             hipLaunchKernelGGL(k_conv_fwd_h<kConvPure>, dim3((unsigned)blocks), dim3(256), 0, st, x, y,
-                               H, tw, g);
+                               H, H16, tw, g);
             break;
         case kConvStorage:
             hipLaunchKernelGGL(k_conv_fwd_h<kConvStorage>, dim3((unsigned)blocks), dim3(256), 0, st, x,
-                               y, H, tw, g);
+                               y, H, H16, tw, g);
             break;
         case kConvMixed:
             hipLaunchKernelGGL(k_conv_fwd_h<kConvMixed>, dim3((unsigned)blocks), dim3(256), 0, st, x, y,
-                               H, tw, g);
+                               H, H16, tw, g);
             break;
         default:
             return hipErrorInvalidValue;
@@ -530,6 +530,13 @@ hipError_t conv_fwd_fp16(int precision, const float* x, float* y, const float2* 
 __global__ void k_zero(float* __restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.f;
+}
+
+// H rounded to FP16 once (the FP16 modes' filter products read it: one
+// 4-byte load per bin instead of an 8-byte load and two conversions)
+__global__ void k_spectrum_h(const float2* __restrict__ H, h2* __restrict__ H16, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) H16[i] = to_h2(H[i]);
 }
 
 // taps (real, K) -> zero-padded complex N
@@ -721,10 +728,15 @@ mifft_status convolver_spectrum(mifft_convolver* cv, const float* taps_dev, hipS
     hipLaunchKernelGGL(k_pack_taps, dim3((n + 255) / 256), dim3(256), 0, st, taps_dev, (int)cv->K,
                        static_cast<float2*>(cv->packed.ptr), n);
     if ((s = check_hip(hipGetLastError(), "pack taps")) != MIFFT_OK) return s;
-    return check_hip(cv->plan->variant->launch(static_cast<const float2*>(cv->packed.ptr),
-                                               static_cast<float2*>(cv->spectrum.ptr),
-                                               cv->plan->twiddles, 1, false, st),
-                     "taps FFT");
+    if ((s = check_hip(cv->plan->variant->launch(static_cast<const float2*>(cv->packed.ptr),
+                                                 static_cast<float2*>(cv->spectrum.ptr),
+                                                 cv->plan->twiddles, 1, false, st),
+                       "taps FFT")) != MIFFT_OK)
+        return s;
+    const float2* H = static_cast<const float2*>(cv->spectrum.ptr);
+    hipLaunchKernelGGL(k_spectrum_h, dim3((n + 255) / 256), dim3(256), 0, st, H,
+                       reinterpret_cast<h2*>(const_cast<float2*>(H) + n), n);
+    return check_hip(hipGetLastError(), "FP16 spectrum");
 }
 
 mifft_status convolver_forward(const mifft_convolver* cv, const float* x, int64_t S, int64_t count,
